@@ -1,0 +1,178 @@
+"""Throughput of the lattice update on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: Taylor-Green vortex, D3Q27, 256^3 cells, periodic, u0 = 0.04, TRT with
+tau+ = 0.6 and magic number 1/4, float populations (the reference's format).  A step is
+one in-place AA update of every cell (one kernel launch).  The K timed steps run
+back-to-back on the lattice's HIP stream with inputs already resident in HBM and are
+timed with HIP events on that stream; MLUPS = cells * K / (1e6 * seconds).
+
+The same line reports the other storage formats (FP16- and BF16-shifted, with their
+error against a double-storage run of the same kernels), D3Q27 BGK, and a CPU baseline:
+the oracle (OpenMP C++ restatement of the reference's kernels) on a bounded sample.
+
+roofline.achieved = 217 B per lattice update (27 float populations read and written
+once, 1 flag byte) * cells / average launch duration.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+B_ALG = {"f32": 2 * 27 * 4 + 1, "f32f": 2 * 27 * 4 + 1, "f16s": 2 * 27 * 2 + 1,
+         "bf16s": 2 * 27 * 2 + 1, "f64": 2 * 27 * 8 + 1}
+
+
+def tgv(n: int, u0: float = 0.04):
+    k = 2 * np.pi / n
+    i = np.arange(n, dtype=np.float64)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    sx, cx = np.sin(k * X), np.cos(k * X)
+    sy, cy = np.sin(k * Y), np.cos(k * Y)
+    cz = np.cos(k * Z)
+    vel = np.empty((n, n, n, 3), np.float32)
+    vel[..., 0] = u0 * sx * cy * cz
+    vel[..., 1] = -u0 * cx * sy * cz
+    vel[..., 2] = 0.0
+    rho = (1 + 3 * (u0 ** 2 / 16) * (np.cos(2 * k * X) + np.cos(2 * k * Y)) *
+           (np.cos(2 * k * Z) + 2)).astype(np.float32)
+    return rho, vel
+
+
+def omega(coll: str, tau: float = 0.6):
+    from boltzmann_amd import core
+    return core.bgk(tau) if coll == "bgk" else core.trt(tau)
+
+
+def run_gpu(n, coll, storage, steps, warmup, rho, vel, keep_vel=False):
+    import boltzmann_amd
+    sim = boltzmann_amd.Simulation("gpu", [n, n, n], omega(coll), storage=storage)
+    sim.fluid.rho[...] = rho
+    sim.fluid.vel[...] = vel
+    sim.step(2)  # first-iterate initialisation + upload, then 2 steps
+    if warmup > 2:
+        sim.step(warmup - warmup % 2)
+    m0 = sim.reduce("mass")
+    t0 = time.perf_counter()
+    ms = sim.step_timed(steps)
+    wall = time.perf_counter() - t0
+    m1 = sim.reduce("mass")
+    cells = n ** 3
+    out = {
+        "mlups": cells * steps / (1e6 * ms * 1e-3),
+        "ms_per_step": ms / steps,
+        "wall_mlups": cells * steps / (1e6 * wall),
+        "mass_drift": abs(m1 - m0) / m0,
+    }
+    if keep_vel:
+        out["vel"] = sim.read_field("vel", dtype="f64")
+    del sim
+    return out
+
+
+def cpu_baseline(budget_s: float = 12.0):
+    """Oracle D3Q27 TRT on a 64^3 TGV sample, all host threads, about budget_s seconds."""
+    import oracle
+    oracle.build()
+    n = 64
+    rho, vel = tgv(n)
+    tau = 0.6
+    om = ("trt", float(np.float32(1 / tau)), float(np.float32(1 / (0.25 / (tau - 0.5) + 0.5))))
+    sim = oracle.OracleSimulation((n, n, n), om)
+    sim.rho[...] = rho
+    sim.vel[...] = vel
+    t0 = time.perf_counter()
+    sim.iterate(2)
+    per = (time.perf_counter() - t0) / 2
+    steps = max(2, int(budget_s / max(per, 1e-6)) // 2 * 2)
+    t0 = time.perf_counter()
+    sim.iterate(steps)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / (1e6 * dt), "unit": "MLUPS", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"oracle (OpenMP C++ restatement of lib.cl) D3Q27 TRT TGV {n}^3, "
+                      f"{steps} steps, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the lattice update here is built and measured on a "
+              "single MI355X (multi-GPU slab decomposition is a design note only).",
+              file=sys.stderr)
+        return 2
+    steps = args.steps + args.steps % 2
+    n = args.n
+    rho, vel = tgv(n)
+    print(f"[bench] TGV D3Q27 {n}^3: building inputs done", file=sys.stderr, flush=True)
+
+    head = run_gpu(n, "trt", "f32", steps, args.warmup, rho, vel)
+    print(f"[bench] f32 trt: {head['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    variants = {}
+    if not args.no_variants:
+        ref64 = run_gpu(n, "trt", "f64", max(2, min(steps, 200)), 0, rho, vel, keep_vel=True)
+        print(f"[bench] f64 trt: {ref64['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        umax = np.abs(ref64["vel"]).max()
+        for coll, st in (("trt", "f16s"), ("trt", "bf16s"), ("bgk", "f32"), ("trt", "f32f")):
+            r = run_gpu(n, coll, st, steps, args.warmup, rho, vel)
+            v = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                 "hbm_frac": round(r["mlups"] * 1e6 * B_ALG[st] / (PEAK_HBM_GBS * 1e9), 4)}
+            if coll == "trt":
+                e = run_gpu(n, coll, st, max(2, min(steps, 200)), 0, rho, vel, keep_vel=True)
+                v["u_err_vs_f64_200steps"] = float(np.abs(e["vel"] - ref64["vel"]).max() / umax)
+            variants[f"{coll}_{st}"] = v
+            print(f"[bench] {coll} {st}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        variants["trt_f64"] = {"mlups": round(ref64["mlups"], 1)}
+        variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
+
+    achieved = head["mlups"] * 1e6 * B_ALG["f32"] / 1e9  # GB/s
+    line = {
+        "metric": "MLUPS & % HBM roofline (D3Q27 FP32/FP16); NN warm-start vs plain time-to-steady",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex D3Q27 TRT {n}^3 periodic, u0=0.04, "
+                               f"tau+=0.6, magic 1/4, FP32 populations, AA in-place",
+                   "cells": n ** 3, "lattice": "D3Q27", "collision": "TRT",
+                   "storage": "f32"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None, "bytes_per_lup": B_ALG["f32"]},
+        "wall_mlups": round(head["wall_mlups"], 1),
+        "mass_drift": head["mass_drift"],
+        "variants": variants,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

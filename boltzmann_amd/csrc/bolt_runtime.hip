@@ -1,0 +1,936 @@
+// libbolt_hip.so: the C ABI of include/bolt.h on HIP for gfx950.
+//
+// Replaces the reference's OpenCL device layer (src/opencl.rs: device selection, context,
+// in-order queue, buffers, kernel registry) and the per-step kernel dispatch of
+// Simulation::iterate (src/simulation.rs:297-433).
+//
+// Device layout (private to this file):
+//   populations  SoA planes, plane i at f + i * stride, stride = cells rounded up to 64
+//   flags        uint8 per cell (the reference's int32 flags use bits 0..3 only)
+//   rho, C       one value per cell (float, or double for double storage)
+//   vel          SoA, vel + a * cells
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bolt.h"
+#include "update.hpp"
+
+using namespace bolt;
+
+// ---------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(BOLT_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------------------
+struct bolt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string info;
+};
+
+struct Tracer {
+    int q = 0;
+    float omega = 0.f;
+    void* g = nullptr;
+    void* C = nullptr;
+};
+
+struct Coupling {
+    int tracer = 0;
+    float alpha = 0.f, c0 = 0.f;
+    float g[3] = {0, 0, 0};
+};
+
+struct bolt_lattice {
+    bolt_ctx* ctx = nullptr;
+    int ndim = 0;
+    int n[3] = {1, 1, 1};
+    long long ncell = 0;
+    long long stride = 0;
+    int q = 0;
+    int collision = 0;
+    float om_p = 0.f, om_n = 0.f;
+    int storage = 0;
+    void* f = nullptr;
+    void* rho = nullptr;
+    void* vel = nullptr;
+    uint8_t* flags = nullptr;
+    std::vector<Tracer> tracers;
+    bool has_gravity = false;
+    float gravity[3] = {0, 0, 0};
+    std::vector<Coupling> couplings;
+    int block = 0;
+    double* partials = nullptr;  // reduction scratch
+    double* result = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t device_bytes = 0;
+};
+
+static size_t store_size(int s) {
+    switch (s) {
+        case kF32:
+        case kF32F: return 4;
+        case kF16S:
+        case kBF16S: return 2;
+        case kF64: return 8;
+    }
+    return 0;
+}
+static bool is_f64(const bolt_lattice* L) { return L->storage == kF64; }
+static size_t macro_size(const bolt_lattice* L) { return is_f64(L) ? 8 : 4; }
+static int tracer_storage(const bolt_lattice* L) { return is_f64(L) ? kF64 : kF32; }
+
+// ---------------------------------------------------------------------------------------
+// kernel table
+// ---------------------------------------------------------------------------------------
+using StepFn = void (*)(StepParams);
+
+template <int D, int Q, int C, int S, int TQ>
+static StepFn pick_even(bool even) {
+    return even ? (StepFn)lbm_step<D, Q, C, S, true, TQ> : (StepFn)lbm_step<D, Q, C, S, false, TQ>;
+}
+
+template <int D, int Q, int C, int S>
+static StepFn step_fn(bool even, int tq) {
+    if (tq == 0) return pick_even<D, Q, C, S, 0>(even);
+    if constexpr (S == kF32 || S == kF64) {
+        constexpr int TQ = D == 2 ? 5 : 7;
+        if (tq == TQ) return pick_even<D, Q, C, S, TQ>(even);
+    }
+    return nullptr;
+}
+
+template <int D, int Q, int C>
+static StepFn step_fn_s(int s, bool even, int tq) {
+    switch (s) {
+        case kF32: return step_fn<D, Q, C, kF32>(even, tq);
+        case kF16S: return step_fn<D, Q, C, kF16S>(even, tq);
+        case kBF16S: return step_fn<D, Q, C, kBF16S>(even, tq);
+        case kF64: return step_fn<D, Q, C, kF64>(even, tq);
+        case kF32F: return step_fn<D, Q, C, kF32F>(even, tq);
+    }
+    return nullptr;
+}
+
+static StepFn lookup_step(int d, int q, int coll, int s, bool even, int tq) {
+    if (d == 2 && q == 9) return coll == 0 ? step_fn_s<2, 9, 0>(s, even, tq) : step_fn_s<2, 9, 1>(s, even, tq);
+    if (d == 3 && q == 27)
+        return coll == 0 ? step_fn_s<3, 27, 0>(s, even, tq) : step_fn_s<3, 27, 1>(s, even, tq);
+    return nullptr;
+}
+
+static StepFn lookup_ad(int d, int tq, int s, bool even) {
+    if (d == 2 && tq == 5) {
+        if (s == kF64) return even ? (StepFn)ad_step<2, 5, kF64, true> : (StepFn)ad_step<2, 5, kF64, false>;
+        return even ? (StepFn)ad_step<2, 5, kF32, true> : (StepFn)ad_step<2, 5, kF32, false>;
+    }
+    if (d == 3 && tq == 7) {
+        if (s == kF64) return even ? (StepFn)ad_step<3, 7, kF64, true> : (StepFn)ad_step<3, 7, kF64, false>;
+        return even ? (StepFn)ad_step<3, 7, kF32, true> : (StepFn)ad_step<3, 7, kF32, false>;
+    }
+    return nullptr;
+}
+
+// ---------------------------------------------------------------------------------------
+// layout / conversion kernels (host AoS <-> device SoA)
+// ---------------------------------------------------------------------------------------
+// weights of slot i for the shift of the reduced-precision formats
+template <int D, int Q>
+__device__ float weight_of(int i) {
+    float w = 0.f;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (k == i) w = VSet<D, Q>::w(k);
+    return w;
+}
+
+__device__ float weight_any(int d, int q, int i) {
+    if (d == 2 && q == 9) return weight_of<2, 9>(i);
+    if (d == 3 && q == 27) return weight_of<3, 27>(i);
+    if (d == 2 && q == 5) return weight_of<2, 5>(i);
+    if (d == 3 && q == 7) return weight_of<3, 7>(i);
+    return 0.f;
+}
+
+template <typename H, int S>
+__global__ void k_pop_to_device(const H* __restrict__ src, long long c0, long long nc, int d,
+                                int q, void* dst, long long stride) {
+    using T = typename Store<S>::T;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc * q) return;
+    const long long cl = k / q;
+    const int i = (int)(k - cl * q);
+    const long long cell = c0 + cl;
+    T* out = (T*)dst + (long long)i * stride + cell;
+    const double v = (double)src[k];
+    if constexpr (S == kF32 || S == kF32F) *out = (float)v;
+    else if constexpr (S == kF64) *out = v;
+    else if constexpr (S == kF16S) *out = (_Float16)((float)v - weight_any(d, q, i));
+    else *out = f32_to_bf16_rne((float)v - weight_any(d, q, i));
+}
+
+template <typename H, int S>
+__global__ void k_pop_to_host(H* __restrict__ dst, long long c0, long long nc, int d, int q,
+                              const void* src, long long stride) {
+    using T = typename Store<S>::T;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc * q) return;
+    const long long cl = k / q;
+    const int i = (int)(k - cl * q);
+    const long long cell = c0 + cl;
+    const T* in = (const T*)src + (long long)i * stride + cell;
+    double v;
+    if constexpr (S == kF32 || S == kF32F || S == kF64) v = (double)*in;
+    else if constexpr (S == kF16S) v = (double)((float)*in + weight_any(d, q, i));
+    else v = (double)(__uint_as_float(((uint32_t)*in) << 16) + weight_any(d, q, i));
+    dst[k] = (H)v;
+}
+
+// per-cell vectors [cells, m] host <-> SoA device (m = 1 for rho/C)
+template <typename H, typename R>
+__global__ void k_vec_to_device(const H* __restrict__ src, long long c0, long long nc, int m,
+                                R* dst, long long ncell) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc * m) return;
+    const long long cl = k / m;
+    const int a = (int)(k - cl * m);
+    dst[(long long)a * ncell + c0 + cl] = (R)src[k];
+}
+
+template <typename H, typename R>
+__global__ void k_vec_to_host(H* __restrict__ dst, long long c0, long long nc, int m,
+                              const R* src, long long ncell) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc * m) return;
+    const long long cl = k / m;
+    const int a = (int)(k - cl * m);
+    dst[k] = (H)src[(long long)a * ncell + c0 + cl];
+}
+
+__global__ void k_flags_to_device(const int32_t* __restrict__ src, long long c0, long long nc,
+                                  uint8_t* dst) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    dst[c0 + k] = (uint8_t)(src[k] & 0xff);
+}
+
+__global__ void k_flags_to_host(int32_t* __restrict__ dst, long long c0, long long nc,
+                                const uint8_t* src) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    dst[k] = (int32_t)src[c0 + k];
+}
+
+template <int S>
+__global__ void k_fill_zero_pops(void* f, long long stride, long long ncell, int d, int q) {
+    using T = typename Store<S>::T;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncell * q) return;
+    const int i = (int)(k / ncell);
+    const long long cell = k - (long long)i * ncell;
+    T* out = (T*)f + (long long)i * stride + cell;
+    if constexpr (S == kF16S) *out = (_Float16)(-weight_any(d, q, i));
+    else *out = f32_to_bf16_rne(-weight_any(d, q, i));
+}
+
+template <typename T>
+__global__ void k_fill(T* p, long long n, T v) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// equilibrium initialisation: velocities.rs:43-52 in float,
+//   f_q = rho * w_q * (1 + 3 vq + 4.5 vq vq - 1.5 vv), vq and vv summed from 0.
+// ---------------------------------------------------------------------------------------
+template <int D, int Q, int S>
+__global__ void k_equilibrate(long long ncell, const void* rho_, const void* vel_, void* f_,
+                              long long stride) {
+#pragma clang fp contract(off)
+    using V = VSet<D, Q>;
+    using T = typename Store<S>::T;
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    T* f = (T*)f_;
+    if constexpr (S == kF64) {
+        const double* rho = (const double*)rho_;
+        const double* vel = (const double*)vel_;
+        double u[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < D; ++a) u[a] = vel[a * ncell + cell];
+        double vv = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) vv += u[a] * u[a];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            double cu = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) cu += (double)V::c(i, a) * u[a];
+            const double w = (double)V::wn(i) / (double)V::wd(i);
+            f[i * stride + cell] = w * rho[cell] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * vv);
+        }
+    } else {
+        const float* rho = (const float*)rho_;
+        const float* vel = (const float*)vel_;
+        float u[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; ++a) u[a] = vel[a * ncell + cell];
+        float vv = 0.0f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) vv = vv + u[a] * u[a];
+        const float r = rho[cell];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            float vq = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) vq = vq + u[a] * (float)V::c(i, a);
+            const float w = V::w(i);
+            if constexpr (S == kF32 || S == kF32F) {
+                f[i * stride + cell] = r * w * (1.0f + 3.0f * vq + 4.5f * vq * vq - 1.5f * vv);
+            } else {
+                // deviation w (rho - 1 + rho (3 vq + 4.5 vq^2 - 1.5 vv)), never forming f
+                const float x = 3.0f * vq + 4.5f * vq * vq - 1.5f * vv;
+                const float dv = w * ((r - 1.0f) + r * x);
+                if constexpr (S == kF16S) f[i * stride + cell] = (_Float16)dv;
+                else f[i * stride + cell] = f32_to_bf16_rne(dv);
+            }
+        }
+    }
+}
+
+template <int D, int Q, int S>
+static void launch_equilibrate(bolt_lattice* L, const void* rho, void* f, long long stride) {
+    const long long nb = (L->ncell + 255) / 256;
+    hipLaunchKernelGGL((k_equilibrate<D, Q, S>), dim3((unsigned)nb), dim3(256), 0, L->ctx->stream,
+                       L->ncell, rho, L->vel, f, stride);
+}
+
+// ---------------------------------------------------------------------------------------
+// reductions (double, deterministic: per-block partials then one block in fixed order)
+// ---------------------------------------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+template <int S>
+__global__ void k_sum_pops(const void* f_, long long stride, long long ncell, int d, int q,
+                           double* partials) {
+    using T = typename Store<S>::T;
+    const T* f = (const T*)f_;
+    double acc = 0.0;
+    const long long total = ncell * q;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ncell);
+        const long long cell = k - (long long)i * ncell;
+        const T v = f[(long long)i * stride + cell];
+        if constexpr (S == kF16S) acc += (double)((float)v) + (double)weight_any(d, q, i);
+        else if constexpr (S == kBF16S)
+            acc += (double)__uint_as_float(((uint32_t)v) << 16) + (double)weight_any(d, q, i);
+        else acc += (double)v;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename R>
+__global__ void k_sum_momentum(const R* rho, const R* vel, const uint8_t* flags, long long ncell,
+                               int a, double* partials) {
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x)
+        if (!(flags[c] & 1)) acc += (double)rho[c] * (double)vel[(long long)a * ncell + c];
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_sum_partials(const double* partials, int n, double* out) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partials[k];
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------------
+static int dev_alloc(bolt_lattice* L, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+        return fail(BOLT_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    L->device_bytes += bytes;
+    return BOLT_OK;
+}
+
+template <typename T>
+static int dev_fill(bolt_lattice* L, void* p, long long n, T v) {
+    if (n <= 0) return BOLT_OK;
+    const long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_fill<T>), dim3((unsigned)nb), dim3(256), 0, L->ctx->stream, (T*)p, n, v);
+    HIP_TRY(hipGetLastError());
+    return BOLT_OK;
+}
+
+static long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
+
+static int pick_block(const bolt_lattice* L) {
+    if (L->block > 0) return L->block;
+    const int len = L->n[L->ndim - 1];
+    const long long rows = L->ncell / len;
+    for (int b : {256, 128}) {
+        const long long nb = rows * ((len + b - 1) / b);
+        if (nb >= 2048 && len >= b) return b;
+    }
+    return 64;
+}
+
+static void fill_geometry(const bolt_lattice* L, StepParams& p, int block) {
+    p.n0 = L->n[0];
+    p.n1 = L->n[1];
+    p.n2 = L->ndim == 3 ? L->n[2] : 1;
+    p.len = L->n[L->ndim - 1];
+    p.cpr = (p.len + block - 1) / block;
+    const long long rows = L->ncell / p.len;
+    p.nblocks = (int)(rows * p.cpr);
+    p.ncell = L->ncell;
+    p.stride = L->stride;
+    p.flags = L->flags;
+}
+
+// ---------------------------------------------------------------------------------------
+// C ABI (C linkage comes from the declarations in bolt.h)
+// ---------------------------------------------------------------------------------------
+
+const char* bolt_last_error(void) { return g_last_error.c_str(); }
+
+const char* bolt_version(void) { return "bolt-hip 0.1 gfx950"; }
+
+int bolt_ctx_create(int device_kind, int device_index, bolt_ctx** out) {
+    if (!out) return fail(BOLT_EINVAL, "out is null");
+    *out = nullptr;
+    if (device_kind != BOLT_DEVICE_GPU)
+        return fail(BOLT_ENOTFOUND,
+                    "no CPU device: this library runs the lattice update on HIP GPUs only");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device_index < 0 || device_index >= n)
+        return fail(BOLT_ENOTFOUND, "no HIP device %d (found %d)", device_index, n);
+    auto c = std::make_unique<bolt_ctx>();
+    c->device = device_index;
+    HIP_TRY(hipSetDevice(device_index));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device_index));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(BOLT_ENOTFOUND, "device %d is %s; this library is built for gfx950",
+                    device_index, prop.gcnArchName);
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->info = std::string("AMD GPU ") + prop.name + " (" + prop.gcnArchName + ")";
+    *out = c.release();
+    return BOLT_OK;
+}
+
+int bolt_ctx_destroy(bolt_ctx* ctx) {
+    if (!ctx) return BOLT_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return BOLT_OK;
+}
+
+int bolt_ctx_device_info(const bolt_ctx* ctx, char* buf, size_t len) {
+    if (!ctx || !buf || len == 0) return fail(BOLT_EINVAL, "bad arguments");
+    snprintf(buf, len, "%s", ctx->info.c_str());
+    return BOLT_OK;
+}
+
+int bolt_lattice_destroy(bolt_lattice* L) {
+    if (!L) return BOLT_OK;
+    hipSetDevice(L->ctx->device);
+    hipStreamSynchronize(L->ctx->stream);
+    hipFree(L->f);
+    hipFree(L->rho);
+    hipFree(L->vel);
+    hipFree(L->flags);
+    for (auto& t : L->tracers) {
+        hipFree(t.g);
+        hipFree(t.C);
+    }
+    hipFree(L->partials);
+    hipFree(L->result);
+    if (L->ev0) hipEventDestroy(L->ev0);
+    if (L->ev1) hipEventDestroy(L->ev1);
+    delete L;
+    return BOLT_OK;
+}
+
+int bolt_lattice_create(bolt_ctx* ctx, int ndim, const int64_t* counts, int q, int collision,
+                        float omega_pos, float omega_neg, int storage, bolt_lattice** out) {
+    if (!ctx || !counts || !out) return fail(BOLT_EINVAL, "null argument");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3)
+        return fail(BOLT_ENOTFOUND, "Kernel not found: D%d (only 2D and 3D lattices)", ndim);
+    if (!((ndim == 2 && q == 9) || (ndim == 3 && q == 27)))
+        return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d NavierStokes", ndim, q);
+    if (collision != BOLT_BGK && collision != BOLT_TRT)
+        return fail(BOLT_EINVAL, "unknown collision %d", collision);
+    if (storage < 0 || storage > BOLT_STORE_F32F)
+        return fail(BOLT_EINVAL, "unknown storage %d", storage);
+    long long ncell = 1;
+    for (int a = 0; a < ndim; ++a) {
+        if (counts[a] < 1 || counts[a] > (1ll << 30))
+            return fail(BOLT_EINVAL, "count[%d] = %lld out of range", a, (long long)counts[a]);
+        ncell *= counts[a];
+    }
+    if (ncell / counts[ndim - 1] > (1ll << 31) / 8)
+        return fail(BOLT_EINVAL, "too many rows for one launch");
+    auto L = std::make_unique<bolt_lattice>();
+    L->ctx = ctx;
+    L->ndim = ndim;
+    for (int a = 0; a < ndim; ++a) L->n[a] = (int)counts[a];
+    L->ncell = ncell;
+    L->stride = round_up(ncell, 64);
+    L->q = q;
+    L->collision = collision;
+    L->om_p = omega_pos;
+    L->om_n = collision == BOLT_TRT ? omega_neg : omega_pos;
+    L->storage = storage;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = dev_alloc(L.get(), &L->f, (size_t)L->stride * q * store_size(storage)))) return rc;
+    if ((rc = dev_alloc(L.get(), &L->rho, (size_t)ncell * macro_size(L.get())))) return rc;
+    if ((rc = dev_alloc(L.get(), &L->vel, (size_t)ncell * ndim * macro_size(L.get())))) return rc;
+    if ((rc = dev_alloc(L.get(), (void**)&L->flags, (size_t)ncell))) return rc;
+    if ((rc = dev_alloc(L.get(), (void**)&L->partials, kRedBlocks * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(L.get(), (void**)&L->result, 8 * sizeof(double)))) return rc;
+    HIP_TRY(hipMemsetAsync(L->f, 0, (size_t)L->stride * q * store_size(storage), ctx->stream));
+    if (storage == kF16S || storage == kBF16S) {
+        // the reference's default f = 0 is the deviation -w_i in the shifted formats
+        const long long n = ncell * q;
+        const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+        if (storage == kF16S)
+            hipLaunchKernelGGL(k_fill_zero_pops<kF16S>, grid, blk, 0, ctx->stream, L->f, L->stride, ncell, ndim, q);
+        else
+            hipLaunchKernelGGL(k_fill_zero_pops<kBF16S>, grid, blk, 0, ctx->stream, L->f, L->stride, ncell, ndim, q);
+        HIP_TRY(hipGetLastError());
+    }
+    if (is_f64(L.get())) {
+        if ((rc = dev_fill<double>(L.get(), L->rho, ncell, 1.0))) return rc;
+    } else {
+        if ((rc = dev_fill<float>(L.get(), L->rho, ncell, 1.0f))) return rc;
+    }
+    HIP_TRY(hipMemsetAsync(L->vel, 0, (size_t)ncell * ndim * macro_size(L.get()), ctx->stream));
+    HIP_TRY(hipMemsetAsync(L->flags, 0, (size_t)ncell, ctx->stream));
+    HIP_TRY(hipEventCreate(&L->ev0));
+    HIP_TRY(hipEventCreate(&L->ev1));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = L.release();
+    return BOLT_OK;
+}
+
+int bolt_lattice_add_tracer(bolt_lattice* L, int q, float omega, int* tracer_id) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    if (!((L->ndim == 2 && q == 5) || (L->ndim == 3 && q == 7)))
+        return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d AdvectionDiffusion BGK", L->ndim, q);
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    Tracer t;
+    t.q = q;
+    t.omega = omega;
+    const size_t es = store_size(tracer_storage(L));
+    int rc;
+    if ((rc = dev_alloc(L, &t.g, (size_t)L->stride * q * es))) return rc;
+    if ((rc = dev_alloc(L, &t.C, (size_t)L->ncell * macro_size(L)))) return rc;
+    HIP_TRY(hipMemsetAsync(t.g, 0, (size_t)L->stride * q * es, L->ctx->stream));
+    HIP_TRY(hipMemsetAsync(t.C, 0, (size_t)L->ncell * macro_size(L), L->ctx->stream));
+    L->tracers.push_back(t);
+    if (tracer_id) *tracer_id = (int)L->tracers.size() - 1;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return BOLT_OK;
+}
+
+int bolt_lattice_set_gravity(bolt_lattice* L, const float* g) {
+    if (!L || !g) return fail(BOLT_EINVAL, "null argument");
+    L->has_gravity = true;
+    for (int a = 0; a < L->ndim; ++a) L->gravity[a] = g[a];
+    return BOLT_OK;
+}
+
+int bolt_lattice_add_coupling(bolt_lattice* L, int tracer_id, float alpha, float c0,
+                              const float* g) {
+    if (!L || !g) return fail(BOLT_EINVAL, "null argument");
+    if (tracer_id < 0 || tracer_id >= (int)L->tracers.size())
+        return fail(BOLT_EINVAL, "unknown tracer %d", tracer_id);
+    Coupling c;
+    c.tracer = tracer_id;
+    c.alpha = alpha;
+    c.c0 = c0;
+    for (int a = 0; a < L->ndim; ++a) c.g[a] = g[a];
+    L->couplings.push_back(c);
+    return BOLT_OK;
+}
+
+// shape of a field: elements per cell and the device pointer
+struct FieldDesc {
+    int per_cell;
+    void* ptr;
+    int kind;  // 0 populations, 1 cell vector, 2 flags
+    int q;
+    int storage;
+};
+
+static int describe(bolt_lattice* L, int field, int tracer_id, FieldDesc& fd) {
+    switch (field) {
+        case BOLT_FIELD_FLAGS: fd = {1, L->flags, 2, 0, 0}; return BOLT_OK;
+        case BOLT_FIELD_F: fd = {L->q, L->f, 0, L->q, L->storage}; return BOLT_OK;
+        case BOLT_FIELD_RHO: fd = {1, L->rho, 1, 0, 0}; return BOLT_OK;
+        case BOLT_FIELD_VEL: fd = {L->ndim, L->vel, 1, 0, 0}; return BOLT_OK;
+        case BOLT_FIELD_G:
+        case BOLT_FIELD_C: {
+            if (tracer_id < 0 || tracer_id >= (int)L->tracers.size())
+                return fail(BOLT_EINVAL, "unknown tracer %d", tracer_id);
+            Tracer& t = L->tracers[tracer_id];
+            if (field == BOLT_FIELD_G) fd = {t.q, t.g, 0, t.q, tracer_storage(L)};
+            else fd = {1, t.C, 1, 0, 0};
+            return BOLT_OK;
+        }
+    }
+    return fail(BOLT_EINVAL, "unknown field %d", field);
+}
+
+template <typename H>
+static void launch_pop_to_device(bolt_lattice* L, const H* src, long long c0, long long nc, int q,
+                                 void* dst, int storage) {
+    const long long n = nc * q;
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    hipStream_t s = L->ctx->stream;
+    switch (storage) {
+        case kF32: hipLaunchKernelGGL((k_pop_to_device<H, kF32>), grid, blk, 0, s, src, c0, nc, L->ndim, q, dst, L->stride); break;
+        case kF32F: hipLaunchKernelGGL((k_pop_to_device<H, kF32F>), grid, blk, 0, s, src, c0, nc, L->ndim, q, dst, L->stride); break;
+        case kF16S: hipLaunchKernelGGL((k_pop_to_device<H, kF16S>), grid, blk, 0, s, src, c0, nc, L->ndim, q, dst, L->stride); break;
+        case kBF16S: hipLaunchKernelGGL((k_pop_to_device<H, kBF16S>), grid, blk, 0, s, src, c0, nc, L->ndim, q, dst, L->stride); break;
+        case kF64: hipLaunchKernelGGL((k_pop_to_device<H, kF64>), grid, blk, 0, s, src, c0, nc, L->ndim, q, dst, L->stride); break;
+    }
+}
+
+template <typename H>
+static void launch_pop_to_host(bolt_lattice* L, H* dst, long long c0, long long nc, int q,
+                               const void* src, int storage) {
+    const long long n = nc * q;
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    hipStream_t s = L->ctx->stream;
+    switch (storage) {
+        case kF32: hipLaunchKernelGGL((k_pop_to_host<H, kF32>), grid, blk, 0, s, dst, c0, nc, L->ndim, q, src, L->stride); break;
+        case kF32F: hipLaunchKernelGGL((k_pop_to_host<H, kF32F>), grid, blk, 0, s, dst, c0, nc, L->ndim, q, src, L->stride); break;
+        case kF16S: hipLaunchKernelGGL((k_pop_to_host<H, kF16S>), grid, blk, 0, s, dst, c0, nc, L->ndim, q, src, L->stride); break;
+        case kBF16S: hipLaunchKernelGGL((k_pop_to_host<H, kBF16S>), grid, blk, 0, s, dst, c0, nc, L->ndim, q, src, L->stride); break;
+        case kF64: hipLaunchKernelGGL((k_pop_to_host<H, kF64>), grid, blk, 0, s, dst, c0, nc, L->ndim, q, src, L->stride); break;
+    }
+}
+
+static size_t host_size(int dt) { return dt == BOLT_DT_F64 ? 8 : 4; }
+
+// Moves a field in chunks of cells through a device staging buffer.
+static int transfer(bolt_lattice* L, int field, int tracer_id, int host_dtype, void* host,
+                    size_t nbytes, bool to_device) {
+    if (!L || !host) return fail(BOLT_EINVAL, "null argument");
+    FieldDesc fd;
+    int rc = describe(L, field, tracer_id, fd);
+    if (rc) return rc;
+    if (fd.kind == 2 && host_dtype != BOLT_DT_I32)
+        return fail(BOLT_EINVAL, "flags are int32 on the host");
+    if (fd.kind != 2 && host_dtype != BOLT_DT_F32 && host_dtype != BOLT_DT_F64)
+        return fail(BOLT_EINVAL, "host dtype must be f32 or f64 for field %d", field);
+    const size_t es = host_size(host_dtype);
+    const size_t want = (size_t)L->ncell * fd.per_cell * es;
+    if (nbytes != want)
+        return fail(BOLT_EINVAL, "field %d needs %zu bytes, got %zu", field, want, nbytes);
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    const size_t stage_bytes = 64u << 20;
+    long long chunk = (long long)(stage_bytes / (fd.per_cell * es));
+    if (chunk > L->ncell) chunk = L->ncell;
+    void* stage = nullptr;
+    HIP_TRY(hipMalloc(&stage, (size_t)chunk * fd.per_cell * es));
+    hipStream_t s = L->ctx->stream;
+    const bool f64m = is_f64(L);
+    int status = BOLT_OK;
+    for (long long c0 = 0; c0 < L->ncell && status == BOLT_OK; c0 += chunk) {
+        const long long nc = (L->ncell - c0 < chunk) ? L->ncell - c0 : chunk;
+        const size_t bytes = (size_t)nc * fd.per_cell * es;
+        char* hp = (char*)host + (size_t)c0 * fd.per_cell * es;
+        const dim3 grid((unsigned)((nc * fd.per_cell + 255) / 256)), blk(256);
+        hipError_t e = hipSuccess;
+        if (to_device) {
+            e = hipMemcpyAsync(stage, hp, bytes, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) {
+                if (fd.kind == 2) {
+                    hipLaunchKernelGGL(k_flags_to_device, grid, blk, 0, s, (const int32_t*)stage, c0, nc, L->flags);
+                } else if (fd.kind == 0) {
+                    if (host_dtype == BOLT_DT_F32) launch_pop_to_device<float>(L, (const float*)stage, c0, nc, fd.q, fd.ptr, fd.storage);
+                    else launch_pop_to_device<double>(L, (const double*)stage, c0, nc, fd.q, fd.ptr, fd.storage);
+                } else {
+                    if (host_dtype == BOLT_DT_F32) {
+                        if (f64m) hipLaunchKernelGGL((k_vec_to_device<float, double>), grid, blk, 0, s, (const float*)stage, c0, nc, fd.per_cell, (double*)fd.ptr, L->ncell);
+                        else hipLaunchKernelGGL((k_vec_to_device<float, float>), grid, blk, 0, s, (const float*)stage, c0, nc, fd.per_cell, (float*)fd.ptr, L->ncell);
+                    } else {
+                        if (f64m) hipLaunchKernelGGL((k_vec_to_device<double, double>), grid, blk, 0, s, (const double*)stage, c0, nc, fd.per_cell, (double*)fd.ptr, L->ncell);
+                        else hipLaunchKernelGGL((k_vec_to_device<double, float>), grid, blk, 0, s, (const double*)stage, c0, nc, fd.per_cell, (float*)fd.ptr, L->ncell);
+                    }
+                }
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+        } else {
+            if (fd.kind == 2) {
+                hipLaunchKernelGGL(k_flags_to_host, grid, blk, 0, s, (int32_t*)stage, c0, nc, L->flags);
+            } else if (fd.kind == 0) {
+                if (host_dtype == BOLT_DT_F32) launch_pop_to_host<float>(L, (float*)stage, c0, nc, fd.q, fd.ptr, fd.storage);
+                else launch_pop_to_host<double>(L, (double*)stage, c0, nc, fd.q, fd.ptr, fd.storage);
+            } else {
+                if (host_dtype == BOLT_DT_F32) {
+                    if (f64m) hipLaunchKernelGGL((k_vec_to_host<float, double>), grid, blk, 0, s, (float*)stage, c0, nc, fd.per_cell, (const double*)fd.ptr, L->ncell);
+                    else hipLaunchKernelGGL((k_vec_to_host<float, float>), grid, blk, 0, s, (float*)stage, c0, nc, fd.per_cell, (const float*)fd.ptr, L->ncell);
+                } else {
+                    if (f64m) hipLaunchKernelGGL((k_vec_to_host<double, double>), grid, blk, 0, s, (double*)stage, c0, nc, fd.per_cell, (const double*)fd.ptr, L->ncell);
+                    else hipLaunchKernelGGL((k_vec_to_host<double, float>), grid, blk, 0, s, (double*)stage, c0, nc, fd.per_cell, (const float*)fd.ptr, L->ncell);
+                }
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(hp, stage, bytes, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+        }
+        if (e != hipSuccess)
+            status = fail(BOLT_EDEVICE, "field transfer failed: %s", hipGetErrorString(e));
+    }
+    hipStreamSynchronize(s);
+    hipFree(stage);
+    return status;
+}
+
+int bolt_lattice_write(bolt_lattice* L, int field, int tracer_id, int host_dtype,
+                       const void* host, size_t nbytes) {
+    return transfer(L, field, tracer_id, host_dtype, (void*)host, nbytes, true);
+}
+
+int bolt_lattice_read(bolt_lattice* L, int field, int tracer_id, int host_dtype, void* host,
+                      size_t nbytes) {
+    return transfer(L, field, tracer_id, host_dtype, host, nbytes, false);
+}
+
+int bolt_lattice_equilibrate(bolt_lattice* L) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    const int s = L->storage;
+#define EQ(D, Q, S) launch_equilibrate<D, Q, S>(L, L->rho, L->f, L->stride)
+    if (L->ndim == 2) {
+        if (s == kF32 || s == kF32F) EQ(2, 9, kF32);
+        else if (s == kF16S) EQ(2, 9, kF16S);
+        else if (s == kBF16S) EQ(2, 9, kBF16S);
+        else EQ(2, 9, kF64);
+    } else {
+        if (s == kF32 || s == kF32F) EQ(3, 27, kF32);
+        else if (s == kF16S) EQ(3, 27, kF16S);
+        else if (s == kBF16S) EQ(3, 27, kBF16S);
+        else EQ(3, 27, kF64);
+    }
+#undef EQ
+    for (auto& t : L->tracers) {
+        if (L->ndim == 2) {
+            if (is_f64(L)) launch_equilibrate<2, 5, kF64>(L, t.C, t.g, L->stride);
+            else launch_equilibrate<2, 5, kF32>(L, t.C, t.g, L->stride);
+        } else {
+            if (is_f64(L)) launch_equilibrate<3, 7, kF64>(L, t.C, t.g, L->stride);
+            else launch_equilibrate<3, 7, kF32>(L, t.C, t.g, L->stride);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return BOLT_OK;
+}
+
+static int enqueue_steps(bolt_lattice* L, uint64_t n) {
+    if (n % 2 != 0) return fail(BOLT_EINVAL, "iters must be even");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    const int block = pick_block(L);
+    StepParams p{};
+    fill_geometry(L, p, block);
+    p.f = L->f;
+    p.rho = L->rho;
+    p.vel = L->vel;
+    p.om_p = L->om_p;
+    p.om_n = L->om_n;
+    // acceleration: the last coupling wins and overrides gravity (simulation.rs:326-373)
+    p.acc_mode = kAccNone;
+    if (!L->couplings.empty()) {
+        const Coupling& c = L->couplings.back();
+        p.acc_mode = kAccBoussinesq;
+        for (int a = 0; a < 3; ++a) p.g[a] = c.g[a];
+        p.alpha = c.alpha;
+        p.c0 = c.c0;
+        p.bq_C = L->tracers[c.tracer].C;
+    } else if (L->has_gravity) {
+        p.acc_mode = kAccConst;
+        for (int a = 0; a < 3; ++a) p.g[a] = L->gravity[a];
+    }
+    // the first tracer is fused into the fluid step when the storage allows it
+    const bool can_fuse = (L->storage == kF32 || L->storage == kF64) && !L->tracers.empty();
+    const int tq = can_fuse ? L->tracers[0].q : 0;
+    if (can_fuse) {
+        p.tg = L->tracers[0].g;
+        p.tC = L->tracers[0].C;
+        p.t_om = L->tracers[0].omega;
+        p.tstride = L->stride;
+    }
+    const size_t first_sep = can_fuse ? 1 : 0;
+    const bool sep_tracers = L->tracers.size() > first_sep;
+    StepFn fe = lookup_step(L->ndim, L->q, L->collision, L->storage, true, tq);
+    StepFn fo = lookup_step(L->ndim, L->q, L->collision, L->storage, false, tq);
+    if (!fe || !fo) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d collision %d storage %d",
+                                L->ndim, L->q, L->collision, L->storage);
+    const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
+    hipStream_t s = L->ctx->stream;
+    for (uint64_t it = 0; it < n; ++it) {
+        const bool even = (it % 2) == 0;
+        p.write_macro = (it + 1 == n) || sep_tracers;
+        hipLaunchKernelGGL(even ? fe : fo, grid, blk, 0, s, p);
+        for (size_t k = first_sep; k < L->tracers.size(); ++k) {
+            const Tracer& t = L->tracers[k];
+            StepParams pt = p;
+            pt.tg = t.g;
+            pt.tC = t.C;
+            pt.t_om = t.omega;
+            pt.tstride = L->stride;
+            StepFn fa = lookup_ad(L->ndim, t.q, tracer_storage(L), even);
+            if (!fa) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d AdvectionDiffusion", L->ndim, t.q);
+            hipLaunchKernelGGL(fa, grid, blk, 0, s, pt);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return BOLT_OK;
+}
+
+int bolt_lattice_step(bolt_lattice* L, uint64_t n) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    return enqueue_steps(L, n);
+}
+
+int bolt_lattice_step_timed(bolt_lattice* L, uint64_t n, double* ms) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    HIP_TRY(hipEventRecord(L->ev0, L->ctx->stream));
+    int rc = enqueue_steps(L, n);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(L->ev1, L->ctx->stream));
+    HIP_TRY(hipEventSynchronize(L->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, L->ev0, L->ev1));
+    if (ms) *ms = t;
+    return BOLT_OK;
+}
+
+int bolt_lattice_sync(bolt_lattice* L) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return BOLT_OK;
+}
+
+static int finish_reduce(bolt_lattice* L, double* out) {
+    hipStream_t s = L->ctx->stream;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, L->partials, kRedBlocks, L->result);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BOLT_OK;
+}
+
+int bolt_lattice_reduce(bolt_lattice* L, int what, int tracer_id, double* out) {
+    if (!L || !out) return fail(BOLT_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    hipStream_t s = L->ctx->stream;
+    const dim3 grid(kRedBlocks), blk(256);
+    if (what == BOLT_REDUCE_MASS || what == BOLT_REDUCE_TRACER) {
+        const void* f = L->f;
+        int q = L->q, st = L->storage;
+        if (what == BOLT_REDUCE_TRACER) {
+            if (tracer_id < 0 || tracer_id >= (int)L->tracers.size())
+                return fail(BOLT_EINVAL, "unknown tracer %d", tracer_id);
+            f = L->tracers[tracer_id].g;
+            q = L->tracers[tracer_id].q;
+            st = tracer_storage(L);
+        }
+        switch (st) {
+            case kF32:
+            case kF32F: hipLaunchKernelGGL(k_sum_pops<kF32>, grid, blk, 0, s, f, L->stride, L->ncell, L->ndim, q, L->partials); break;
+            case kF16S: hipLaunchKernelGGL(k_sum_pops<kF16S>, grid, blk, 0, s, f, L->stride, L->ncell, L->ndim, q, L->partials); break;
+            case kBF16S: hipLaunchKernelGGL(k_sum_pops<kBF16S>, grid, blk, 0, s, f, L->stride, L->ncell, L->ndim, q, L->partials); break;
+            case kF64: hipLaunchKernelGGL(k_sum_pops<kF64>, grid, blk, 0, s, f, L->stride, L->ncell, L->ndim, q, L->partials); break;
+        }
+        HIP_TRY(hipGetLastError());
+        return finish_reduce(L, out);
+    }
+    if (what == BOLT_REDUCE_MOMENTUM) {
+        for (int a = 0; a < L->ndim; ++a) {
+            if (is_f64(L))
+                hipLaunchKernelGGL(k_sum_momentum<double>, grid, blk, 0, s, (const double*)L->rho, (const double*)L->vel, L->flags, L->ncell, a, L->partials);
+            else
+                hipLaunchKernelGGL(k_sum_momentum<float>, grid, blk, 0, s, (const float*)L->rho, (const float*)L->vel, L->flags, L->ncell, a, L->partials);
+            HIP_TRY(hipGetLastError());
+            int rc = finish_reduce(L, out + a);
+            if (rc) return rc;
+        }
+        return BOLT_OK;
+    }
+    return fail(BOLT_EINVAL, "unknown reduction %d", what);
+}
+
+int bolt_lattice_device_bytes(const bolt_lattice* L, uint64_t* out) {
+    if (!L || !out) return fail(BOLT_EINVAL, "null argument");
+    *out = L->device_bytes;
+    return BOLT_OK;
+}
+
+int bolt_lattice_set_block(bolt_lattice* L, int block) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    if (block != 0 && (block < 64 || block > 256 || block % 64 != 0))
+        return fail(BOLT_EINVAL, "block must be 64, 128, 192 or 256 (or 0)");
+    L->block = block;
+    return BOLT_OK;
+}

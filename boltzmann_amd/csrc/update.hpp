@@ -1,0 +1,688 @@
+// AA-pattern in-place stream + collide for gfx950.
+//
+// One work-item per cell.  Populations are structure-of-arrays: plane i holds slot i of
+// every cell, f[i * stride + cell], with cell = (ix*ny + iy)*nz + iz (2D: ix*ny + iy), the
+// reference's row-major cell order (lib.cl:204,426).  The last axis is contiguous, so the
+// 64 lanes of a wavefront read and write 64 consecutive cells of one plane.
+//
+// AA addressing (lib.cl:226-246, 276-296):
+//   even step  load  f_i  = F[x - c_i, i]          store F[x - c_i, i]  = f*_{opp i}
+//   odd step   load  f_i  = F[x, opp i]            store F[x, i]        = f*_i
+// Every work-item reads and writes the same set of locations and the sets of different
+// cells are disjoint, so one launch updates the array in place with no ordering between
+// work-items.  Wall cells (flag & 1) neither read nor write (full-way bounce-back).
+//
+// Per-cell physics (lib.cl:249-300, 680-717; kernel_gen.py:402-420 for D3Q27-TRT):
+//   rho = sum f, u = sum c f / rho, u += a / omega(+) when an acceleration is active,
+//   fixed fluid cells (flag & 6) relax fully (omega = 1) towards feq(rho[x], vel[x]),
+//   BGK:  f += omega (feq - f);   TRT: symmetric half with omega+, antisymmetric with omega-.
+// Advection-diffusion tracer (lib.cl:102-189, 303-405): rho = sum g, u = the fluid's u of
+// the same step, fixed scalar cells (flag & 8) relax fully towards geq(C[x], u).
+//
+// Arithmetic: storage kF32 evaluates each equilibrium polynomial, the relaxation and the
+// population update in double and narrows to float exactly where the reference's kernels
+// do (its unsuffixed literals are double); the operation order is the reference's and
+// multiply-adds are not contracted, so results are bit-identical to the CPU restatement.
+// Storage kF32F/kF16S/kBF16S use float arithmetic (the shifted formats on deviations
+// f_i - w_i); kF64 uses double throughout.
+#pragma once
+
+#include "lattice.hpp"
+
+namespace bolt {
+
+struct StepParams {
+    int n0, n1, n2;      // counts; 2D runs have n2 == 1 and the contiguous axis is n1
+    int len;             // length of the contiguous axis
+    int cpr;             // work-groups per row of the contiguous axis
+    int nblocks;         // work-groups in the launch
+    long long ncell;
+    long long stride;    // elements between two population planes of the fluid
+    void* f;
+    void* rho;           // float, or double for kF64
+    void* vel;           // SoA: vel[a * ncell + cell]
+    const uint8_t* flags;
+    float om_p, om_n;
+    int acc_mode;        // 0: none, 1: constant g, 2: Boussinesq from bq_C
+    float g[3];
+    float alpha, c0;
+    const void* bq_C;    // tracer values read for the buoyancy (previous step)
+    int write_macro;     // write rho/vel of non-fixed cells this step
+    // fused advection-diffusion tracer
+    void* tg;
+    void* tC;
+    float t_om;
+    long long tstride;
+};
+
+enum AccMode : int { kAccNone = 0, kAccConst = 1, kAccBoussinesq = 2 };
+
+// Work-groups are dealt to the 8 XCDs round-robin (XCD = id % 8).  Renumber them so that
+// each XCD works on one contiguous range of rows: neighbouring work-groups then share an
+// L2 for the cache lines straddling their boundary.
+__device__ inline int xcd_swizzle(int p, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int k = p & 7, j = p >> 3;
+    return k * q + (k < r ? k : r) + j;
+}
+
+// Cell coordinates and periodic neighbour indices of the cell owned by this work-item.
+template <int D>
+struct Nbr;
+
+template <>
+struct Nbr<2> {
+    long long xb[3];  // (ix + dx mod nx) * ny for dx = -1, 0, 1
+    int y[3];         // iy + dy mod ny
+    __device__ Nbr(int ix, int iy, const StepParams& p) {
+        const int xm = ix == 0 ? p.n0 - 1 : ix - 1, xp = ix == p.n0 - 1 ? 0 : ix + 1;
+        xb[0] = (long long)xm * p.n1;
+        xb[1] = (long long)ix * p.n1;
+        xb[2] = (long long)xp * p.n1;
+        y[0] = iy == 0 ? p.n1 - 1 : iy - 1;
+        y[1] = iy;
+        y[2] = iy == p.n1 - 1 ? 0 : iy + 1;
+    }
+    __device__ long long at(int dx, int dy, int) const { return xb[dx + 1] + y[dy + 1]; }
+    __device__ long long self() const { return xb[1] + y[1]; }
+};
+
+template <>
+struct Nbr<3> {
+    long long rb[3][3];  // ((ix+dx)*ny + (iy+dy)) * nz
+    int z[3];
+    __device__ Nbr(int ix, int iy, int iz, const StepParams& p) {
+        const int xs[3] = {ix == 0 ? p.n0 - 1 : ix - 1, ix, ix == p.n0 - 1 ? 0 : ix + 1};
+        const int ys[3] = {iy == 0 ? p.n1 - 1 : iy - 1, iy, iy == p.n1 - 1 ? 0 : iy + 1};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                rb[a][b] = ((long long)xs[a] * p.n1 + ys[b]) * (long long)p.n2;
+        z[0] = iz == 0 ? p.n2 - 1 : iz - 1;
+        z[1] = iz;
+        z[2] = iz == p.n2 - 1 ? 0 : iz + 1;
+    }
+    __device__ long long at(int dx, int dy, int dz) const { return rb[dx + 1][dy + 1] + z[dz + 1]; }
+    __device__ long long self() const { return rb[1][1] + z[1]; }
+};
+
+// ---- AA loads and stores ---------------------------------------------------------------
+template <class V, bool EVEN, class T, class NB>
+__device__ inline void aa_load_raw(const T* __restrict__ f, long long stride, const NB& nb,
+                                   long long cell, T (&v)[V::q]) {
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        if (EVEN)
+            v[i] = f[i * stride + nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))];
+        else
+            v[i] = f[V::opp(i) * stride + cell];
+    }
+}
+
+template <class V, bool EVEN, class T, class NB>
+__device__ inline void aa_store_raw(T* __restrict__ f, long long stride, const NB& nb,
+                                    long long cell, const T (&v)[V::q]) {
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        if (EVEN)
+            f[i * stride + nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))] = v[V::opp(i)];
+        else
+            f[i * stride + cell] = v[i];
+    }
+}
+
+// ---- reference arithmetic (storage kF32) -------------------------------------------------
+// Equilibrium polynomial of slot i, double, in the reference's printed term order.
+template <class V, int I>
+__device__ inline double ref_feq_poly(const float (&u)[3], float vv) {
+#pragma clang fp contract(off)
+    constexpr double c_vv = (-3.0 * V::wn(I)) / (2.0 * V::wd(I));
+    constexpr double c_lin = (3.0 * V::wn(I)) / (double)V::wd(I);
+    constexpr double c_sq = (9.0 * V::wn(I)) / (2.0 * V::wd(I));
+    constexpr double c_0 = (double)V::wn(I) / (double)V::wd(I);
+    if constexpr (V::nnz(I) == 0) {
+        return c_0 + c_vv * (double)vv;
+    } else {
+        float cu = 0.0f;
+        bool first = true;
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) {
+            if (V::c(I, a) == 0) continue;
+            const float t = V::c(I, a) > 0 ? u[a] : -u[a];
+            cu = first ? t : cu + t;
+            first = false;
+        }
+        const float sq = cu * cu;
+        double p = c_vv * (double)vv;
+        if constexpr (V::nnz(I) == 1) {
+            p = p + c_sq * (double)sq;
+#pragma unroll
+            for (int a = 0; a < V::d; ++a)
+                if (V::c(I, a) != 0) p = p + (V::c(I, a) > 0 ? c_lin : -c_lin) * (double)u[a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < V::d; ++a)
+                if (V::c(I, a) != 0) p = p + (V::c(I, a) > 0 ? c_lin : -c_lin) * (double)u[a];
+            p = p + c_sq * (double)sq;
+        }
+        return p + c_0;
+    }
+}
+
+template <class V, int I>
+__device__ inline double ref_feq_pos(const float (&u)[3], float vv) {
+#pragma clang fp contract(off)
+    constexpr double c_vv = (-3.0 * V::wn(I)) / (2.0 * V::wd(I));
+    constexpr double c_sq = (9.0 * V::wn(I)) / (2.0 * V::wd(I));
+    constexpr double c_0 = (double)V::wn(I) / (double)V::wd(I);
+    if constexpr (V::nnz(I) == 0) {
+        return c_0 + c_vv * (double)vv;
+    } else {
+        float cu = 0.0f;
+        bool first = true;
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) {
+            if (V::c(I, a) == 0) continue;
+            const float t = V::c(I, a) > 0 ? u[a] : -u[a];
+            cu = first ? t : cu + t;
+            first = false;
+        }
+        const float sq = cu * cu;
+        double p = c_vv * (double)vv;
+        p = p + c_sq * (double)sq;
+        return p + c_0;
+    }
+}
+
+template <class V, int I>
+__device__ inline double ref_feq_neg(const float (&u)[3]) {
+#pragma clang fp contract(off)
+    constexpr double c_lin = (3.0 * V::wn(I)) / (double)V::wd(I);
+    double p = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) {
+        if (V::c(I, a) == 0) continue;
+        const double t = (V::c(I, a) > 0 ? c_lin : -c_lin) * (double)u[a];
+        p = first ? t : p + t;
+        first = false;
+    }
+    return p;
+}
+
+template <class V, int I = 0>
+__device__ inline void ref_bgk(float (&f)[V::q], const float (&u)[3], float vv, float r,
+                               float om) {
+#pragma clang fp contract(off)
+    if constexpr (I < V::q) {
+        const double p = ref_feq_poly<V, I>(u, vv);
+        const double upd = (double)om * ((double)r * p - (double)f[I]);
+        f[I] = (float)((double)f[I] + upd);
+        ref_bgk<V, I + 1>(f, u, vv, r, om);
+    }
+}
+
+template <class V, int I = 0>
+__device__ inline void ref_trt(float (&f)[V::q], const float (&u)[3], float vv, float r,
+                               float omp, float omn) {
+#pragma clang fp contract(off)
+    if constexpr (I < V::q) {
+        constexpr int J = V::opp(I);
+        if constexpr (J >= I) {
+            const double pp = ref_feq_pos<V, I>(u, vv);
+            const double pn = ref_feq_neg<V, I>(u);
+            const float fpos =
+                (float)((double)omp * ((double)r * pp - (double)(f[I] + f[J]) * 0.5));
+            const float fneg =
+                (float)((double)omn * ((double)r * pn - (double)(f[I] - f[J]) * 0.5));
+            f[I] = f[I] + (fpos + fneg);
+            if constexpr (J != I) f[J] = f[J] + (fpos - fneg);
+        }
+        ref_trt<V, I + 1>(f, u, vv, r, omp, omn);
+    }
+}
+
+// rho and u in the reference's order.
+template <class V>
+__device__ inline void ref_moments(const float (&f)[V::q], float& r, float (&u)[3]) {
+#pragma clang fp contract(off)
+    r = f[0];
+#pragma unroll
+    for (int i = 1; i < V::q; ++i) r = r + f[i];
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) {
+        float s = 0.0f;
+        bool first = true;
+#pragma unroll
+        for (int k = 0; k < V::q; ++k) {
+            const int i = vel_order<V::q>(k);
+            if (V::c(i, a) == 0) continue;
+            const float t = V::c(i, a) > 0 ? f[i] : -f[i];
+            s = first ? t : s + t;
+            first = false;
+        }
+        u[a] = s / r;
+    }
+}
+
+template <int D>
+__device__ inline float ref_vv(const float (&u)[3]) {
+#pragma clang fp contract(off)
+    float vv = u[0] * u[0];
+    vv = vv + u[1] * u[1];
+    if (D == 3) vv = vv + u[2] * u[2];
+    return vv;
+}
+
+// ---- float arithmetic (kF32F, on populations) ------------------------------------------
+template <class V>
+__device__ inline void f32_moments(const float (&f)[V::q], float& r, float (&u)[3]) {
+    r = 0.f;
+    float j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        r += f[i];
+#pragma unroll
+        for (int a = 0; a < V::d; ++a)
+            if (V::c(i, a) != 0) j[a] += V::c(i, a) > 0 ? f[i] : -f[i];
+    }
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) u[a] = j[a] / r;
+}
+
+template <class V>
+__device__ inline float cdotu(int i, const float (&u)[3]) {
+    float cu = 0.f;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a)
+        if (V::c(i, a) != 0) cu += V::c(i, a) > 0 ? u[a] : -u[a];
+    return cu;
+}
+
+template <class V>
+__device__ inline void f32_bgk(float (&f)[V::q], const float (&u)[3], float vv, float r,
+                               float om) {
+    const float base = 1.0f - 1.5f * vv;
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        const float cu = cdotu<V>(i, u);
+        const float feq = (V::w(i) * r) * (base + cu * (3.0f + 4.5f * cu));
+        f[i] += om * (feq - f[i]);
+    }
+}
+
+template <class V>
+__device__ inline void f32_trt(float (&f)[V::q], const float (&u)[3], float vv, float r,
+                               float omp, float omn) {
+    const float base = 1.0f - 1.5f * vv;
+    f[0] += omp * ((V::w(0) * r) * base - f[0]);
+#pragma unroll
+    for (int i = 1; i < V::q; i += 2) {
+        const int j = i + 1;
+        const float cu = cdotu<V>(i, u);
+        const float wr = V::w(i) * r;
+        const float eqp = wr * (base + 4.5f * cu * cu);
+        const float eqm = wr * (3.0f * cu);
+        const float dp = omp * (eqp - 0.5f * (f[i] + f[j]));
+        const float dm = omn * (eqm - 0.5f * (f[i] - f[j]));
+        f[i] += dp + dm;
+        f[j] += dp - dm;
+    }
+}
+
+// ---- float arithmetic on deviations d_i = f_i - w_i (shifted storage) -------------------
+// rho - 1 = sum d, j = sum c d (the w_i cancel pairwise), and the update
+//   d_i += omega (w_i (rho - 1) + w_i rho (3 cu + 9/2 cu^2 - 3/2 u.u) - d_i)
+// never forms w_i + d_i, which keeps the small deviations exact to float rounding.
+template <class V>
+__device__ inline void dev_moments(const float (&d)[V::q], float& dr, float (&u)[3]) {
+    dr = 0.f;
+    float j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        dr += d[i];
+#pragma unroll
+        for (int a = 0; a < V::d; ++a)
+            if (V::c(i, a) != 0) j[a] += V::c(i, a) > 0 ? d[i] : -d[i];
+    }
+    const float r = 1.0f + dr;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) u[a] = j[a] / r;
+}
+
+template <class V>
+__device__ inline void dev_bgk(float (&d)[V::q], const float (&u)[3], float vv, float dr,
+                               float om) {
+    const float r = 1.0f + dr;
+    const float m15vv = -1.5f * vv;
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        const float cu = cdotu<V>(i, u);
+        const float deq = V::w(i) * (dr + r * (m15vv + cu * (3.0f + 4.5f * cu)));
+        d[i] += om * (deq - d[i]);
+    }
+}
+
+template <class V>
+__device__ inline void dev_trt(float (&d)[V::q], const float (&u)[3], float vv, float dr,
+                               float omp, float omn) {
+    const float r = 1.0f + dr;
+    const float m15vv = -1.5f * vv;
+    d[0] += omp * (V::w(0) * (dr + r * m15vv) - d[0]);
+#pragma unroll
+    for (int i = 1; i < V::q; i += 2) {
+        const int j = i + 1;
+        const float cu = cdotu<V>(i, u);
+        const float eqp = V::w(i) * (dr + r * (m15vv + 4.5f * cu * cu));
+        const float eqm = V::w(i) * (r * 3.0f * cu);
+        const float dp = omp * (eqp - 0.5f * (d[i] + d[j]));
+        const float dm = omn * (eqm - 0.5f * (d[i] - d[j]));
+        d[i] += dp + dm;
+        d[j] += dp - dm;
+    }
+}
+
+// ---- double arithmetic (kF64) -------------------------------------------------------------
+template <class V>
+__device__ inline void f64_step(double (&f)[V::q], double& r, double (&u)[3], bool fixed,
+                                int coll, const StepParams& p, double rfix,
+                                const double (&ufix)[3], const double (&acc)[3], bool use_acc) {
+#pragma clang fp contract(off)
+    r = 0.0;
+    u[0] = u[1] = u[2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        r += f[i];
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) u[a] += (double)V::c(i, a) * f[i];
+    }
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) u[a] /= r;
+    double op = p.om_p, on = p.om_n;
+    if (use_acc)
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) u[a] += acc[a] / op;
+    if (fixed) {
+        op = 1.0;
+        on = 1.0;
+        r = rfix;
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) u[a] = ufix[a];
+    }
+    double vv = 0.0;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) vv += u[a] * u[a];
+    double feq[V::q];
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        double cu = 0.0;
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) cu += (double)V::c(i, a) * u[a];
+        const double w = (double)V::wn(i) / (double)V::wd(i);
+        feq[i] = w * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * vv);
+    }
+    if (coll == 0) {
+#pragma unroll
+        for (int i = 0; i < V::q; ++i) f[i] += op * (feq[i] - f[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < V::q; ++i) {
+            const int j = V::opp(i);
+            if (j < i) continue;
+            const double fpos = op * (0.5 * (feq[i] + feq[j]) - 0.5 * (f[i] + f[j]));
+            const double fneg = on * (0.5 * (feq[i] - feq[j]) - 0.5 * (f[i] - f[j]));
+            f[i] += fpos + fneg;
+            if (j != i) f[j] += fpos - fneg;
+        }
+    }
+}
+
+template <class V>
+__device__ inline void f64_ad(double (&g)[V::q], double& r, const double (&u)[3], bool fixed,
+                              double om, double cfix) {
+#pragma clang fp contract(off)
+    r = 0.0;
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) r += g[i];
+    if (fixed) {
+        om = 1.0;
+        r = cfix;
+    }
+    double vv = 0.0;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) vv += u[a] * u[a];
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) {
+        double cu = 0.0;
+#pragma unroll
+        for (int a = 0; a < V::d; ++a) cu += (double)V::c(i, a) * u[a];
+        const double w = (double)V::wn(i) / (double)V::wd(i);
+        g[i] += om * (w * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * vv) - g[i]);
+    }
+}
+
+// ---- advection-diffusion update of one cell, float storage, reference arithmetic --------
+template <class V>
+__device__ inline void ref_ad(float (&g)[V::q], float& r, const float (&u)[3], bool fixed,
+                              float om, float cfix) {
+#pragma clang fp contract(off)
+    r = g[0];
+#pragma unroll
+    for (int i = 1; i < V::q; ++i) r = r + g[i];
+    if (fixed) {
+        om = 1.0f;
+        r = cfix;
+    }
+    const float vv = ref_vv<V::d>(u);
+    ref_bgk<V>(g, u, vv, r, om);
+}
+
+// Maps the work-item to its cell.  Returns false for work-items past the end of a row.
+template <int D>
+__device__ inline bool locate(const StepParams& p, int& ix, int& iy, int& iz, long long& cell) {
+    const int b = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    const int row = b / p.cpr;
+    const int chunk = b - row * p.cpr;
+    const int zc = chunk * (int)blockDim.x + (int)threadIdx.x;
+    if (zc >= p.len) return false;
+    if (D == 2) {
+        ix = row;
+        iy = zc;
+        iz = 0;
+    } else {
+        ix = row / p.n1;
+        iy = row - ix * p.n1;
+        iz = zc;
+    }
+    cell = (long long)row * p.len + zc;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// The fluid step, optionally fused with one advection-diffusion tracer (TQ = 5 or 7).
+// ---------------------------------------------------------------------------------------
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
+__global__ void __launch_bounds__(256) lbm_step(const StepParams p) {
+    using V = VSet<D, Q>;
+    using ST = Store<S>;
+    using T = typename ST::T;
+    int ix, iy, iz;
+    long long cell;
+    if (!locate<D>(p, ix, iy, iz, cell)) return;
+    const int flag = p.flags[cell];
+    if (flag & 1) return;
+    const bool fixed = (flag & 6) != 0;
+    Nbr<D> nb = [&]() {
+        if constexpr (D == 2) return Nbr<2>(ix, iy, p);
+        else return Nbr<3>(ix, iy, iz, p);
+    }();
+    T* __restrict__ fp = (T*)p.f;
+
+    if constexpr (S == kF64) {
+        double f[Q];
+        aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+        const double* rho = (const double*)p.rho;
+        const double* vel = (const double*)p.vel;
+        double rfix = 0.0, ufix[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
+        if (fixed) {
+            rfix = rho[cell];
+#pragma unroll
+            for (int a = 0; a < D; ++a) ufix[a] = vel[a * p.ncell + cell];
+        }
+        const bool use_acc = p.acc_mode != kAccNone;
+        if (p.acc_mode == kAccConst) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] = (double)p.g[a];
+        } else if (p.acc_mode == kAccBoussinesq) {
+            const double b = (double)p.alpha * (((const double*)p.bq_C)[cell] - (double)p.c0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] = (double)p.g[a] * b;
+        }
+        double r, u[3];
+        f64_step<V>(f, r, u, fixed, COLL, p, rfix, ufix, acc, use_acc);
+        aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+        if (p.write_macro && !fixed) {
+            ((double*)p.rho)[cell] = r;
+#pragma unroll
+            for (int a = 0; a < D; ++a) ((double*)p.vel)[a * p.ncell + cell] = u[a];
+        }
+        if constexpr (TQ != 0) {
+            using W = VSet<D, TQ>;
+            double* gp = (double*)p.tg;
+            double g[TQ];
+            aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            const bool tfixed = (flag & 8) != 0;
+            double* C = (double*)p.tC;
+            double rt;
+            f64_ad<W>(g, rt, u, tfixed, (double)p.t_om, tfixed ? C[cell] : 0.0);
+            aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            C[cell] = rt;
+        }
+    } else {
+        float* rho = (float*)p.rho;
+        float* vel = (float*)p.vel;
+        float f[Q];
+        float r, u[3] = {0.f, 0.f, 0.f};
+        float omp = p.om_p, omn = p.om_n;
+        if constexpr (S == kF32 || S == kF32F) {
+            aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+            if constexpr (S == kF32) ref_moments<V>(f, r, u);
+            else f32_moments<V>(f, r, u);
+        } else {
+            T raw[Q];
+            aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, raw);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if constexpr (S == kF16S) f[i] = (float)raw[i];
+                else f[i] = __uint_as_float(((uint32_t)raw[i]) << 16);
+            }
+            dev_moments<V>(f, r, u);  // r holds rho - 1 here
+        }
+        if (p.acc_mode != kAccNone) {
+            float acc[3];
+            if (p.acc_mode == kAccConst) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] = p.g[a];
+            } else {
+                const float b = p.alpha * (((const float*)p.bq_C)[cell] - p.c0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] = p.g[a] * b;
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) u[a] = u[a] + acc[a] / omp;
+        }
+        if (fixed) {
+            omp = 1.0f;
+            omn = 1.0f;
+            r = rho[cell];
+            if constexpr (S == kF16S || S == kBF16S) r = r - 1.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) u[a] = vel[a * p.ncell + cell];
+        }
+        if constexpr (S == kF32) {
+            const float vv = ref_vv<D>(u);
+            if constexpr (COLL == 0) ref_bgk<V>(f, u, vv, r, omp);
+            else ref_trt<V>(f, u, vv, r, omp, omn);
+            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+        } else if constexpr (S == kF32F) {
+            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            if constexpr (COLL == 0) f32_bgk<V>(f, u, vv, r, omp);
+            else f32_trt<V>(f, u, vv, r, omp, omn);
+            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+        } else {
+            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
+            else dev_trt<V>(f, u, vv, r, omp, omn);
+            T raw[Q];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if constexpr (S == kF16S) raw[i] = (_Float16)f[i];
+                else raw[i] = f32_to_bf16_rne(f[i]);
+            }
+            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, raw);
+            r = r + 1.0f;
+        }
+        if (p.write_macro && !fixed) {
+            rho[cell] = r;
+#pragma unroll
+            for (int a = 0; a < D; ++a) vel[a * p.ncell + cell] = u[a];
+        }
+        if constexpr (TQ != 0) {
+            static_assert(S == kF32, "fused tracers use float storage");
+            using W = VSet<D, TQ>;
+            float* gp = (float*)p.tg;
+            float g[TQ];
+            aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            const bool tfixed = (flag & 8) != 0;
+            float* C = (float*)p.tC;
+            float rt;
+            ref_ad<W>(g, rt, u, tfixed, p.t_om, tfixed ? C[cell] : 0.0f);
+            aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            C[cell] = rt;
+        }
+    }
+}
+
+// Advection-diffusion tracer on its own, reading the fluid velocity written this step.
+template <int D, int TQ, int S, bool EVEN>
+__global__ void __launch_bounds__(256) ad_step(const StepParams p) {
+    using W = VSet<D, TQ>;
+    int ix, iy, iz;
+    long long cell;
+    if (!locate<D>(p, ix, iy, iz, cell)) return;
+    const int flag = p.flags[cell];
+    if (flag & 1) return;
+    const bool tfixed = (flag & 8) != 0;
+    Nbr<D> nb = [&]() {
+        if constexpr (D == 2) return Nbr<2>(ix, iy, p);
+        else return Nbr<3>(ix, iy, iz, p);
+    }();
+    if constexpr (S == kF64) {
+        double* gp = (double*)p.tg;
+        double g[TQ];
+        aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        double u[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < D; ++a) u[a] = ((const double*)p.vel)[a * p.ncell + cell];
+        double* C = (double*)p.tC;
+        double rt;
+        f64_ad<W>(g, rt, u, tfixed, (double)p.t_om, tfixed ? C[cell] : 0.0);
+        aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        C[cell] = rt;
+    } else {
+        float* gp = (float*)p.tg;
+        float g[TQ];
+        aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        float u[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; ++a) u[a] = ((const float*)p.vel)[a * p.ncell + cell];
+        float* C = (float*)p.tC;
+        float rt;
+        ref_ad<W>(g, rt, u, tfixed, p.t_om, tfixed ? C[cell] : 0.0f);
+        aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        C[cell] = rt;
+    }
+}
+
+}  // namespace bolt

@@ -1,0 +1,152 @@
+/*
+ * bolt.h -- C ABI of the MI355X lattice update (libbolt_hip.so).
+ *
+ * This is the boundary a host program binds in place of the reference's OpenCL layer:
+ * the device/context/queue/buffer code of src/opencl.rs and the per-step kernel dispatch
+ * of src/simulation.rs (djbarker/boltzmann).  Host state -- the host arrays, the
+ * iteration counter, the first-iteration equilibrium, checkpoints -- stays with the
+ * caller, as it does in the reference's Simulation (simulation.rs:130-192).
+ *
+ * Conventions
+ *  - Every function returns BOLT_OK (0) or a BOLT_E* code; bolt_last_error() then holds a
+ *    message for the calling thread.  (The reference panics instead, opencl.rs:109-120.)
+ *  - Host arrays are the reference's layout: row-major over the cell counts with x
+ *    slowest, per-cell components last (f[..., q], vel[..., d]).  Device layout is private.
+ *  - A lattice is used by one thread at a time; distinct lattices may run concurrently.
+ *  - No callbacks, no global state besides the per-thread error message.
+ */
+#ifndef BOLT_H
+#define BOLT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define BOLT_OK 0
+#define BOLT_EINVAL 1     /* bad argument (odd step count, wrong size, unknown field) */
+#define BOLT_ENOTFOUND 2  /* no kernel for this (D, Q, equation, collision, storage) */
+#define BOLT_EDEVICE 3    /* HIP runtime error */
+#define BOLT_ENOMEM 4     /* device allocation failed */
+#define BOLT_ESTATE 5     /* call not valid in the current state */
+
+/* ---- enumerations ---------------------------------------------------------------- */
+/* device kinds, as the reference's DeviceType (opencl.rs:20-33) */
+#define BOLT_DEVICE_CPU 0 /* not provided by this library: returns BOLT_ENOTFOUND */
+#define BOLT_DEVICE_GPU 1
+
+/* collision operators, as Omega::BGK / Omega::TRT (fields.rs:37-41) */
+#define BOLT_BGK 0
+#define BOLT_TRT 1
+
+/* population storage */
+#define BOLT_STORE_F32 0   /* float; equilibrium evaluated as the reference does   */
+#define BOLT_STORE_F16S 1  /* binary16 of f_i - w_i, float arithmetic              */
+#define BOLT_STORE_BF16S 2 /* bfloat16 of f_i - w_i, float arithmetic              */
+#define BOLT_STORE_F64 3   /* double storage and arithmetic                        */
+#define BOLT_STORE_F32F 4  /* float storage, plain float arithmetic                */
+
+/* fields (the reference's Data<T,D> buffers, fields.rs:55-214, simulation.rs:22-59) */
+#define BOLT_FIELD_FLAGS 0 /* int32 [cells]       cell flags (simulation.rs:80-89)  */
+#define BOLT_FIELD_F 1     /* [cells, Q]          fluid populations                 */
+#define BOLT_FIELD_RHO 2   /* [cells]             fluid density                     */
+#define BOLT_FIELD_VEL 3   /* [cells, D]          fluid velocity                    */
+#define BOLT_FIELD_G 4     /* [cells, Qt]         tracer populations                */
+#define BOLT_FIELD_C 5     /* [cells]             tracer value                      */
+
+/* host element types for bolt_lattice_write / bolt_lattice_read */
+#define BOLT_DT_F32 0
+#define BOLT_DT_F64 1
+#define BOLT_DT_I32 2
+
+/* reductions, summed in double on the device */
+#define BOLT_REDUCE_MASS 0      /* sum of every population slot, walls included: 1 value */
+#define BOLT_REDUCE_MOMENTUM 1  /* sum over non-wall cells of rho*u: D values          */
+#define BOLT_REDUCE_TRACER 2    /* sum of every tracer population slot: 1 value       */
+
+typedef struct bolt_ctx bolt_ctx;
+typedef struct bolt_lattice bolt_lattice;
+
+/* Message for the last failed call on this thread ("" if none). */
+const char* bolt_last_error(void);
+
+/* Library version string, e.g. "bolt-hip 0.1 gfx950". */
+const char* bolt_version(void);
+
+/* ---- context: replaces OpenCLCtx::new / DeviceType (opencl.rs:20-33, 93-171) -------- */
+/* device_kind: BOLT_DEVICE_GPU.  device_index: HIP ordinal.  Creates one in-order stream. */
+int bolt_ctx_create(int device_kind, int device_index, bolt_ctx** out);
+int bolt_ctx_destroy(bolt_ctx* ctx);
+/* "<vendor> <GPU|CPU> <name>", the reference's device_info getter (lib.rs:205-215). */
+int bolt_ctx_device_info(const bolt_ctx* ctx, char* buf, size_t len);
+
+/* ---- lattice: the device buffers of one Simulation (Cells + Fluid + Scalars) ---------
+ * Replaces Cells::new / Fluid::new / Scalar::new device allocation (simulation.rs:32-42,
+ * fields.rs:68-82,150-163) and the kernel registry lookup make_kernel_key
+ * (fields.rs:24-34, opencl.rs:123-161).  Supported (D, Q): (2, 9), (3, 27) with BGK or TRT
+ * (the reference has no D3Q27-TRT kernel; this library adds it).  Device buffers start as
+ * the reference's defaults: f = 0, rho = 1, vel = 0, flags = 0. */
+int bolt_lattice_create(bolt_ctx* ctx, int ndim, const int64_t* counts, int q, int collision,
+                        float omega_pos, float omega_neg, int storage, bolt_lattice** out);
+int bolt_lattice_destroy(bolt_lattice* lat);
+
+/* Adds an advection-diffusion tracer, Scalar::new + Simulation::add_tracer
+ * (fields.rs:150-163, simulation.rs:215-223).  q: 5 in 2D, 7 in 3D; BGK only, as in the
+ * reference (a TRT tracer has no kernel there either).  C = 0, g = 0 initially. */
+int bolt_lattice_add_tracer(bolt_lattice* lat, int q, float omega, int* tracer_id);
+
+/* Constant acceleration g[ndim], Simulation::set_gravity (simulation.rs:194-196). */
+int bolt_lattice_set_gravity(bolt_lattice* lat, const float* g);
+
+/* Boussinesq coupling acc = g * (alpha * (C_tracer - c0)), add_boussinesq_coupling
+ * (simulation.rs:201-213).  As in the reference, couplings overwrite the acceleration each
+ * step, the last one added wins, and any coupling overrides gravity (simulation.rs:326-373). */
+int bolt_lattice_add_coupling(bolt_lattice* lat, int tracer_id, float alpha, float c0,
+                              const float* g);
+
+/* Host -> device copy of a whole field, Data::enqueue_write (opencl.rs:253-266).
+ * tracer_id selects the tracer for BOLT_FIELD_G / BOLT_FIELD_C, ignored otherwise.
+ * nbytes must equal elements * sizeof(host type). */
+int bolt_lattice_write(bolt_lattice* lat, int field, int tracer_id, int host_dtype,
+                       const void* host, size_t nbytes);
+
+/* Device -> host copy of a whole field, Data::enqueue_read (opencl.rs:268-281).
+ * Synchronous: returns once the host buffer holds the data. */
+int bolt_lattice_read(bolt_lattice* lat, int field, int tracer_id, int host_dtype, void* host,
+                      size_t nbytes);
+
+/* Device-side equilibrium initialisation: f = feq(rho, vel) and, for every tracer,
+ * g = geq(C, vel), in float with the reference's host formula (velocities.rs:43-52). */
+int bolt_lattice_equilibrate(bolt_lattice* lat);
+
+/* Runs n time steps (n even), the per-step dispatch of Simulation::iterate
+ * (simulation.rs:297-433): Boussinesq acceleration -> fluid update -> tracer updates.
+ * rho/vel/C hold the last step's values when the call returns.  Asynchronous with respect
+ * to the host; bolt_lattice_read and bolt_lattice_sync wait. */
+int bolt_lattice_step(bolt_lattice* lat, uint64_t n);
+
+/* As bolt_lattice_step, and reports the device time of the n steps in milliseconds,
+ * measured with HIP events on the lattice's stream.  Synchronous. */
+int bolt_lattice_step_timed(bolt_lattice* lat, uint64_t n, double* ms);
+
+/* Waits for all work queued on the lattice's stream (queue.finish()). */
+int bolt_lattice_sync(bolt_lattice* lat);
+
+/* Double-precision reductions (see BOLT_REDUCE_*); out receives 1 or D values. */
+int bolt_lattice_reduce(bolt_lattice* lat, int what, int tracer_id, double* out);
+
+/* Device bytes held by the lattice's buffers. */
+int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);
+
+/* Launch tuning: work-group size along the contiguous axis (64..1024, multiple of 64);
+ * 0 restores the built-in choice. */
+int bolt_lattice_set_block(bolt_lattice* lat, int block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOLT_H */

@@ -1,0 +1,351 @@
+"""Parity of the HIP lattice update with the oracle, through the product API.
+
+Float storage (kF32) evaluates the collision exactly as the reference's kernels do (double
+equilibrium narrowed to float, same operation order, no contraction), so populations,
+densities, velocities and tracer values are compared BIT-EXACTLY with the oracle on the
+same seeded inputs, for every (D, Q, collision), with walls, fixed cells, gravity,
+fused and separate tracers and Boussinesq couplings, on ragged sizes.  The reference's
+1e-5 relative max-norm bar is checked as well on the Kelvin-Helmholtz workload, and mass
+conservation in FP64 on large domains.
+"""
+
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_built():
+    oracle.build()
+
+
+def _bolt():
+    import boltzmann_amd
+    from boltzmann_amd import core
+    return boltzmann_amd, core
+
+
+def omega_obj(om):
+    _, core = _bolt()
+    if om[0] == "bgk":
+        return core.Omega.BGK(om[1])
+    return core.Omega.TRT(om[1], om[2])
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def mixed_flags(counts, rng, walls=0.12, fixed=0.05, scalar=0.0):
+    flags = np.zeros(counts, np.int32)
+    w = rng.random(counts) < walls
+    flags[w] = 1
+    fx = (rng.random(counts) < fixed) & ~w
+    flags[fx] = rng.choice([2, 4, 6], size=int(fx.sum()))
+    if scalar:
+        flags[(rng.random(counts) < scalar) & ~w] |= 8
+    flags |= (rng.random(counts) < 0.05).astype(np.int32) * 32  # ignored bits
+    return flags
+
+
+def make_pair(counts, om, q=None, storage="f32", seed=0, walls=0.12, fixed=0.05,
+              gravity=None, tracers=(), couplings=(), scalar=0.0, vel_scale=0.04):
+    """Builds the product Simulation and the oracle with identical inputs."""
+    bolt, core = _bolt()
+    rng = np.random.default_rng(seed)
+    d = len(counts)
+    flags = mixed_flags(counts, rng, walls, fixed, scalar)
+    rho = (1 + 0.02 * rng.standard_normal(counts)).astype(np.float32)
+    vel = (vel_scale * rng.standard_normal(counts + (d,))).astype(np.float32)
+    sim = bolt.Simulation("gpu", list(counts), omega_obj(om), q, storage=storage)
+    ref = oracle.OracleSimulation(counts, om, q=q)
+    sim.cells.flags[...] = flags
+    sim.fluid.rho[...] = rho
+    sim.fluid.vel[...] = vel
+    ref.flags[...] = flags
+    ref.rho[...] = rho
+    ref.vel[...] = vel
+    for name, tom in tracers:
+        C = rng.random(counts).astype(np.float32)
+        tr = sim.add_tracer(name, core.Omega.BGK(tom))
+        tr.val[...] = C
+        ref.add_tracer(name, tom).C[...] = C
+    if gravity is not None:
+        g = np.asarray(gravity, np.float32)
+        sim.set_gravity(g)
+        ref.set_gravity(g)
+    for name, alpha, c0, g in couplings:
+        g = np.asarray(g, np.float32)
+        sim.add_boussinesq_coupling(sim.get_tracer(name), alpha, c0, g)
+        ref.add_boussinesq_coupling(name, alpha, c0, g)
+    return sim, ref, flags
+
+
+def assert_same(sim, ref, flags):
+    nw = (flags & 1) == 0
+    f = sim.read_field("f")
+    np.testing.assert_array_equal(f, ref.f)
+    np.testing.assert_array_equal(sim.fluid.rho[nw], ref.rho[nw])
+    np.testing.assert_array_equal(sim.fluid.vel[nw], ref.vel[nw])
+    for name, t in ref.tracers.items():
+        np.testing.assert_array_equal(sim.read_field("g", name), t.g)
+        np.testing.assert_array_equal(sim.get_tracer(name).val[nw], t.C[nw])
+
+
+SETS = [
+    ((37, 29), 9, ("bgk", f32(1 / 0.6))),
+    ((37, 29), 9, ("trt", f32(1 / 0.55), f32(1 / (0.25 / 0.05 + 0.5)))),
+    ((9, 11, 70), 27, ("bgk", f32(1 / 0.7))),
+    ((9, 11, 70), 27, ("trt", f32(1 / 0.6), f32(1 / (0.25 / 0.1 + 0.5)))),
+]
+
+
+@pytest.mark.parametrize("counts,q,om", SETS)
+def test_omega_zero_is_bit_exact_permutation(counts, q, om):
+    om0 = ("bgk", 0.0) if om[0] == "bgk" else ("trt", 0.0, 0.0)
+    sim, ref, flags = make_pair(counts, om0, q, fixed=0.0, seed=1)
+    for _ in range(3):
+        sim.iterate(2)
+        ref.iterate(2)
+        assert_same(sim, ref, flags)
+
+
+@pytest.mark.parametrize("counts,q,om", SETS)
+@pytest.mark.parametrize("extra", ["plain", "gravity", "tracer", "coupling", "two_tracers"])
+def test_f32_bit_exact_against_oracle(counts, q, om, extra):
+    d = len(counts)
+    kw = {}
+    tq = f32(1 / 0.8)
+    if extra == "gravity":
+        kw["gravity"] = [1e-4, -2e-4, 5e-5][:d]
+    if extra in ("tracer", "coupling", "two_tracers"):
+        kw["tracers"] = [("T", tq)]
+        kw["scalar"] = 0.05
+    if extra == "two_tracers":
+        kw["tracers"] = [("T", tq), ("S", f32(1 / 0.9))]
+        kw["couplings"] = [("S", 0.1, 0.5, [0.0, -1.0, 0.0][:d])]
+    if extra == "coupling":
+        kw["gravity"] = [0.3, 0.3, 0.3][:d]  # overridden by the coupling
+        kw["couplings"] = [("T", 1e-3, 0.5, [0.0, -1.0, 0.0][:d])]
+    sim, ref, flags = make_pair(counts, om, q, seed=2, **kw)
+    for n in (2, 4, 6):
+        sim.iterate(n)
+        ref.iterate(n)
+        assert_same(sim, ref, flags)
+
+
+@pytest.mark.parametrize("counts", [(1, 7), (5, 1), (2, 2), (1, 1, 5), (3, 1, 2), (4, 3, 1)])
+def test_degenerate_sizes_bit_exact(counts):
+    q = 9 if len(counts) == 2 else 27
+    om = ("bgk", f32(1 / 0.7)) if q == 9 else ("trt", f32(1 / 0.7), f32(1 / 0.9))
+    sim, ref, flags = make_pair(counts, om, q, seed=3, walls=0.2)
+    sim.iterate(4)
+    ref.iterate(4)
+    assert_same(sim, ref, flags)
+
+
+def test_all_walls_nothing_moves():
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [16, 20], core.bgk(0.6))
+    sim.cells.flags[...] = 1
+    sim.fluid.vel[..., 0] = 0.03
+    sim.iterate(2)
+    f0 = sim.read_field("f")
+    sim.iterate(10)
+    np.testing.assert_array_equal(sim.read_field("f"), f0)
+    np.testing.assert_array_equal(sim.fluid.vel[..., 0], np.float32(0.03))
+
+
+def kh_inputs(n=256, seed=0):
+    """Kelvin-Helmholtz shear layers, D2Q9 BGK tau 0.52 (the reference's CPU-runnable case)."""
+    x = np.arange(n)[:, None] * np.ones((1, n))
+    y = np.ones((n, 1)) * np.arange(n)[None, :]
+    ux = 0.05 * np.tanh((y - 64) / 4) - 0.05 * np.tanh((y - 192) / 4)
+    uy = 1e-3 * np.sin(2 * np.pi * x / n)
+    rng = np.random.default_rng(seed)
+    vel = np.stack([ux, uy], -1) + rng.normal(0.0, 1e-4, (n, n, 2))
+    return vel.astype(np.float32)
+
+
+def test_kelvin_helmholtz_parity_and_mass():
+    bolt, core = _bolt()
+    vel = kh_inputs()
+    om = f32(1 / 0.52)
+    sim = bolt.Simulation("gpu", [256, 256], core.Omega.BGK(om))
+    ref = oracle.OracleSimulation((256, 256), ("bgk", om))
+    sim.fluid.vel[...] = vel
+    ref.vel[...] = vel
+    done = 0
+    m0 = None
+    for target in (2, 10, 100, 1000):
+        sim.iterate(target - done)
+        ref.iterate(target - done)
+        done = target
+        f = sim.read_field("f")
+        rel = np.abs(f - ref.f).max() / np.abs(ref.f).max()
+        assert rel <= 1e-5, (target, rel)
+        np.testing.assert_array_equal(f, ref.f)
+        m = sim.reduce("mass")
+        if m0 is None:
+            m0 = m
+        assert abs(m - m0) / m0 <= 1e-6
+        np.testing.assert_allclose(m, ref.f.astype(np.float64).sum(), rtol=1e-12)
+
+
+# ---- the reference's integration tests (python/tests/test_core.py:57-127) on the GPU ----
+
+def test_simulation_1():  # test_core.py:57-79
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [100, 100], core.bgk(0.51))
+    assert sim.fluid.vel.shape == (100, 100, 2)
+    assert sim.fluid.rho.shape == (100, 100)
+    assert sim.cells.flags.shape == (100, 100)
+    assert abs(sim.fluid.vel.min() - 0) < 1e-8
+    assert abs(sim.fluid.vel.max() - 0) < 1e-8
+    assert abs(sim.fluid.rho.max() - 1) < 1e-8
+    sim.iterate(10)
+    assert abs(sim.fluid.vel.mean() - 0) < 1e-4
+    assert abs(sim.fluid.rho.mean() - 1) < 1e-4
+
+
+def test_simulation_2():  # test_core.py:82-93
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [100, 100], core.bgk(0.51))
+    tracer = sim.add_tracer("tracer", core.bgk(0.51))
+    tracer.val[10:20, 30:40] = 1.0
+    sim.iterate(10)
+    assert abs(tracer.val.sum() - 10 ** 2) < 1e-4
+
+
+def test_simulation_3():  # test_core.py:96-103
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [100, 100], core.bgk(1.0))
+    sim.set_gravity(np.array([0, 0.1], dtype=np.float32))
+    sim.iterate(10)
+    assert abs(sim.fluid.vel[..., 1].mean() - 0.1 * 10) < 1e-4
+
+
+def test_simulation_serde():  # test_core.py:106-127
+    bolt, core = _bolt()
+    with tempfile.TemporaryDirectory() as tmpdirname:
+        path = tmpdirname + "/checkpoint.mpk"
+        sim1 = bolt.Simulation("gpu", [100, 100], core.bgk(0.51))
+        trc1 = sim1.add_tracer("tracer", core.bgk(0.51))
+        trc1.val[10:20, 30:40] = 1.0
+        sim1.add_boussinesq_coupling(trc1, 0.1, 0, np.array([0, 1], np.float32))
+        sim1.iterate(10)
+        sim1.write_checkpoint(path)
+        sim2 = bolt.Simulation.load_checkpoint("gpu", path)
+        trc2 = sim2.get_tracer("tracer")
+        assert np.allclose(sim1.fluid.rho, sim2.fluid.rho)
+        assert np.allclose(sim1.fluid.vel, sim2.fluid.vel)
+        assert np.allclose(trc1.val, trc2.val)
+        # and the restored run continues bit-identically
+        sim1.iterate(6)
+        sim2.iterate(6)
+        np.testing.assert_array_equal(sim1.read_field("f"), sim2.read_field("f"))
+        np.testing.assert_array_equal(trc1.val, trc2.val)
+        assert sim2.iteration == 16
+
+
+def test_odd_iterate_rejected_and_host_edits_after_start_ignored():
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [16, 16], core.bgk(0.6))
+    with pytest.raises(bolt.BoltError, match="even"):
+        sim.iterate(3)
+    sim.fluid.vel[..., 0] = 0.02 * np.sin(np.arange(16) * 2 * np.pi / 16)[None, :]
+    sim.iterate(2)
+    f0 = sim.read_field("f")
+    sim.cells.flags[...] = 1  # uploaded only at iteration 0 (simulation.rs:283-286)
+    sim.iterate(2)
+    # had the all-wall flags reached the device, nothing would have moved
+    assert not np.array_equal(sim.read_field("f"), f0)
+    assert sim.iteration == 4
+
+
+def test_3d_gravity_kat_trt():
+    bolt, core = _bolt()
+    sim = bolt.Simulation("gpu", [6, 6, 70], core.Omega.TRT(1.0, 0.7))
+    sim.set_gravity(np.array([0, 0, 0.05], np.float32))
+    sim.iterate(6)
+    a = np.float32(0.05)
+    assert abs(sim.fluid.vel[..., 2].mean() - (5 * 0.7 * a + a)) < 1e-5
+
+
+# ---- double storage and the reduced-precision formats ----------------------------------
+
+@pytest.mark.parametrize("counts,q,om", SETS)
+def test_f64_storage_matches_oracle_f64(counts, q, om):
+    bolt, core = _bolt()
+    rng = np.random.default_rng(4)
+    d = len(counts)
+    flags = mixed_flags(counts, rng)
+    rho = (1 + 0.02 * rng.standard_normal(counts)).astype(np.float32)
+    vel = (0.04 * rng.standard_normal(counts + (d,))).astype(np.float32)
+    sim = bolt.Simulation("gpu", list(counts), omega_obj(om), q, storage="f64")
+    ref = oracle.OracleSimulation(counts, om, q=q, dtype=np.float64)
+    sim.cells.flags[...] = flags
+    sim.fluid.rho[...] = rho
+    sim.fluid.vel[...] = vel
+    ref.flags[...] = flags
+    ref.rho[...] = rho
+    ref.vel[...] = vel
+    sim.iterate(6)
+    ref.iterate(6)
+    np.testing.assert_allclose(sim.read_field("f", dtype="f64"), ref.f, rtol=0, atol=1e-13)
+
+
+def tgv_inputs(n, u0=0.04):
+    k = 2 * np.pi / n
+    i = np.arange(n)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    u = u0 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
+    v = -u0 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
+    w = np.zeros_like(u)
+    rho = 1 + 3 * (u0 ** 2 / 16) * (np.cos(2 * k * X) + np.cos(2 * k * Y)) * (np.cos(2 * k * Z) + 2)
+    return rho.astype(np.float32), np.stack([u, v, w], -1).astype(np.float32)
+
+
+@pytest.mark.parametrize("coll", ["bgk", "trt"])
+def test_reduced_precision_against_f64_run(coll):
+    bolt, core = _bolt()
+    n = 32
+    rho, vel = tgv_inputs(n)
+    om = core.Omega.BGK(1 / 0.6) if coll == "bgk" else core.Omega.TRT(1 / 0.6, 1 / (0.25 / 0.1 + 0.5))
+    out = {}
+    for st in ("f64", "f32", "f32f", "f16s", "bf16s"):
+        sim = bolt.Simulation("gpu", [n, n, n], om, storage=st)
+        sim.fluid.rho[...] = rho
+        sim.fluid.vel[...] = vel
+        sim.iterate(100)
+        out[st] = sim.read_field("vel", dtype="f64")
+        m = sim.reduce("mass")
+        assert abs(m - rho.astype(np.float64).sum()) / (n ** 3) < {"f64": 1e-12, "f32": 1e-6,
+                                                                   "f32f": 1e-6, "f16s": 1e-4,
+                                                                   "bf16s": 1e-3}[st]
+    umax = np.abs(out["f64"]).max()
+    err = {k: np.abs(v - out["f64"]).max() / umax for k, v in out.items()}
+    assert err["f32"] < 1e-5 and err["f32f"] < 1e-5
+    assert err["f16s"] < 2e-3
+    assert err["bf16s"] < 3e-2
+
+
+def test_mass_conservation_large_d3q27():
+    bolt, core = _bolt()
+    n = 128
+    rho, vel = tgv_inputs(n)
+    sim = bolt.Simulation("gpu", [n, n, n], core.Omega.TRT(1 / 0.51, 1 / (0.25 / 0.01 + 0.5)))
+    sim.fluid.rho[...] = rho
+    sim.fluid.vel[...] = vel
+    sim.iterate(2)
+    m0 = sim.reduce("mass")
+    sim.iterate(100)
+    m1 = sim.reduce("mass")
+    assert abs(m1 - m0) / m0 < 1e-6
+    assert np.isfinite(sim.fluid.vel).all()
