@@ -429,9 +429,18 @@ static void fill_geometry(const bolt_lattice* L, StepParams& p, int block) {
     p.cpr = (p.len + block - 1) / block;
     const long long rows = L->ncell / p.len;
     p.nblocks = (int)(rows * p.cpr);
-    p.ncell = L->ncell;
-    p.stride = L->stride;
     p.flags = L->flags;
+}
+
+// plane i of a population array starts i * stride elements in
+static void fill_planes(const bolt_lattice* L, void* base, int q, int storage, void** planes) {
+    const size_t es = store_size(storage);
+    for (int i = 0; i < q; ++i) planes[i] = (char*)base + (size_t)i * L->stride * es;
+}
+
+static void fill_vel(const bolt_lattice* L, StepParams& p) {
+    for (int a = 0; a < 3; ++a)
+        p.vel[a] = a < L->ndim ? (char*)L->vel + (size_t)a * L->ncell * macro_size(L) : nullptr;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -523,6 +532,12 @@ int bolt_lattice_create(bolt_ctx* ctx, int ndim, const int64_t* counts, int q, i
     }
     if (ncell / counts[ndim - 1] > (1ll << 31) / 8)
         return fail(BOLT_EINVAL, "too many rows for one launch");
+    // kernels address every array with 32-bit byte offsets per cell
+    const long long widest = (storage == BOLT_STORE_F64) ? 8 : 4;
+    if (ncell * widest >= (1ll << 32))
+        return fail(BOLT_EINVAL,
+                    "%lld cells exceed one lattice (at most 2^32 bytes per population plane: "
+                    "%lld cells in this storage)", ncell, ((1ll << 32) - 1) / widest);
     auto L = std::make_unique<bolt_lattice>();
     L->ctx = ctx;
     L->ndim = ndim;
@@ -791,9 +806,9 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const int block = pick_block(L);
     StepParams p{};
     fill_geometry(L, p, block);
-    p.f = L->f;
+    fill_planes(L, L->f, L->q, L->storage, p.fpl);
     p.rho = L->rho;
-    p.vel = L->vel;
+    fill_vel(L, p);
     p.om_p = L->om_p;
     p.om_n = L->om_n;
     // acceleration: the last coupling wins and overrides gravity (simulation.rs:326-373)
@@ -813,10 +828,9 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const bool can_fuse = (L->storage == kF32 || L->storage == kF64) && !L->tracers.empty();
     const int tq = can_fuse ? L->tracers[0].q : 0;
     if (can_fuse) {
-        p.tg = L->tracers[0].g;
+        fill_planes(L, L->tracers[0].g, L->tracers[0].q, tracer_storage(L), p.gpl);
         p.tC = L->tracers[0].C;
         p.t_om = L->tracers[0].omega;
-        p.tstride = L->stride;
     }
     const size_t first_sep = can_fuse ? 1 : 0;
     const bool sep_tracers = L->tracers.size() > first_sep;
@@ -833,10 +847,9 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
         for (size_t k = first_sep; k < L->tracers.size(); ++k) {
             const Tracer& t = L->tracers[k];
             StepParams pt = p;
-            pt.tg = t.g;
+            fill_planes(L, t.g, t.q, tracer_storage(L), pt.gpl);
             pt.tC = t.C;
             pt.t_om = t.omega;
-            pt.tstride = L->stride;
             StepFn fa = lookup_ad(L->ndim, t.q, tracer_storage(L), even);
             if (!fa) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d AdvectionDiffusion", L->ndim, t.q);
             hipLaunchKernelGGL(fa, grid, blk, 0, s, pt);

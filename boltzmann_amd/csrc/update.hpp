@@ -23,24 +23,26 @@
 // population update in double and narrows to float exactly where the reference's kernels
 // do (its unsuffixed literals are double); the operation order is the reference's and
 // multiply-adds are not contracted, so results are bit-identical to the CPU restatement.
-// Storage kF32F/kF16S/kBF16S use float arithmetic (the shifted formats on deviations
-// f_i - w_i); kF64 uses double throughout.
+// Storage kF32F/kF16S/kBF16S use float arithmetic on the deviations f_i - w_i; kF64 uses
+// double throughout.
 #pragma once
 
 #include "lattice.hpp"
 
 namespace bolt {
 
+// Kernel arguments.  Every array is addressed as a uniform base pointer plus a 32-bit
+// per-cell byte offset (cell * sizeof(element) < 2^32, enforced at lattice creation), so
+// loads and stores use the scalar-base form of the global memory instructions and keep no
+// 64-bit address in vector registers.
 struct StepParams {
     int n0, n1, n2;      // counts; 2D runs have n2 == 1 and the contiguous axis is n1
     int len;             // length of the contiguous axis
     int cpr;             // work-groups per row of the contiguous axis
     int nblocks;         // work-groups in the launch
-    long long ncell;
-    long long stride;    // elements between two population planes of the fluid
-    void* f;
+    void* fpl[27];       // population planes of the fluid: slot i of cell c at fpl[i][c]
     void* rho;           // float, or double for kF64
-    void* vel;           // SoA: vel[a * ncell + cell]
+    void* vel[3];        // velocity component planes
     const uint8_t* flags;
     float om_p, om_n;
     int acc_mode;        // 0: none, 1: constant g, 2: Boussinesq from bq_C
@@ -48,12 +50,20 @@ struct StepParams {
     float alpha, c0;
     const void* bq_C;    // tracer values read for the buoyancy (previous step)
     int write_macro;     // write rho/vel of non-fixed cells this step
-    // fused advection-diffusion tracer
-    void* tg;
+    // advection-diffusion tracer (fused into lbm_step, or the one ad_step updates)
+    void* gpl[7];
     void* tC;
     float t_om;
-    long long tstride;
 };
+
+template <class T>
+__device__ inline T& elem(void* base, uint32_t cell) {
+    return *(T*)((char*)base + cell * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ inline const T& elem(const void* base, uint32_t cell) {
+    return *(const T*)((const char*)base + cell * (uint32_t)sizeof(T));
+}
 
 enum AccMode : int { kAccNone = 0, kAccConst = 1, kAccBoussinesq = 2 };
 
@@ -72,63 +82,62 @@ struct Nbr;
 
 template <>
 struct Nbr<2> {
-    long long xb[3];  // (ix + dx mod nx) * ny for dx = -1, 0, 1
-    int y[3];         // iy + dy mod ny
-    __device__ Nbr(int ix, int iy, const StepParams& p) {
+    uint32_t xb[3];  // (ix + dx mod nx) * ny for dx = -1, 0, 1
+    uint32_t y[3];   // iy + dy mod ny
+    __device__ Nbr(int ix, int iy, int, const StepParams& p) {
         const int xm = ix == 0 ? p.n0 - 1 : ix - 1, xp = ix == p.n0 - 1 ? 0 : ix + 1;
-        xb[0] = (long long)xm * p.n1;
-        xb[1] = (long long)ix * p.n1;
-        xb[2] = (long long)xp * p.n1;
+        xb[0] = (uint32_t)xm * (uint32_t)p.n1;
+        xb[1] = (uint32_t)ix * (uint32_t)p.n1;
+        xb[2] = (uint32_t)xp * (uint32_t)p.n1;
         y[0] = iy == 0 ? p.n1 - 1 : iy - 1;
         y[1] = iy;
         y[2] = iy == p.n1 - 1 ? 0 : iy + 1;
     }
-    __device__ long long at(int dx, int dy, int) const { return xb[dx + 1] + y[dy + 1]; }
-    __device__ long long self() const { return xb[1] + y[1]; }
+    __device__ uint32_t at(int dx, int dy, int) const { return xb[dx + 1] + y[dy + 1]; }
 };
 
 template <>
 struct Nbr<3> {
-    long long rb[3][3];  // ((ix+dx)*ny + (iy+dy)) * nz
-    int z[3];
+    uint32_t rb[3][3];  // ((ix+dx)*ny + (iy+dy)) * nz
+    uint32_t z[3];
     __device__ Nbr(int ix, int iy, int iz, const StepParams& p) {
-        const int xs[3] = {ix == 0 ? p.n0 - 1 : ix - 1, ix, ix == p.n0 - 1 ? 0 : ix + 1};
-        const int ys[3] = {iy == 0 ? p.n1 - 1 : iy - 1, iy, iy == p.n1 - 1 ? 0 : iy + 1};
+        const uint32_t xs[3] = {(uint32_t)(ix == 0 ? p.n0 - 1 : ix - 1), (uint32_t)ix,
+                                (uint32_t)(ix == p.n0 - 1 ? 0 : ix + 1)};
+        const uint32_t ys[3] = {(uint32_t)(iy == 0 ? p.n1 - 1 : iy - 1), (uint32_t)iy,
+                                (uint32_t)(iy == p.n1 - 1 ? 0 : iy + 1)};
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b)
-                rb[a][b] = ((long long)xs[a] * p.n1 + ys[b]) * (long long)p.n2;
+            for (int b = 0; b < 3; ++b) rb[a][b] = (xs[a] * (uint32_t)p.n1 + ys[b]) * (uint32_t)p.n2;
         z[0] = iz == 0 ? p.n2 - 1 : iz - 1;
         z[1] = iz;
         z[2] = iz == p.n2 - 1 ? 0 : iz + 1;
     }
-    __device__ long long at(int dx, int dy, int dz) const { return rb[dx + 1][dy + 1] + z[dz + 1]; }
-    __device__ long long self() const { return rb[1][1] + z[1]; }
+    __device__ uint32_t at(int dx, int dy, int dz) const { return rb[dx + 1][dy + 1] + z[dz + 1]; }
 };
 
 // ---- AA loads and stores ---------------------------------------------------------------
 template <class V, bool EVEN, class T, class NB>
-__device__ inline void aa_load_raw(const T* __restrict__ f, long long stride, const NB& nb,
-                                   long long cell, T (&v)[V::q]) {
+__device__ inline void aa_load_raw(void* const* planes, const NB& nb, uint32_t cell,
+                                   T (&v)[V::q]) {
 #pragma unroll
     for (int i = 0; i < V::q; ++i) {
         if (EVEN)
-            v[i] = f[i * stride + nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))];
+            v[i] = elem<T>((const void*)planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)));
         else
-            v[i] = f[V::opp(i) * stride + cell];
+            v[i] = elem<T>((const void*)planes[V::opp(i)], cell);
     }
 }
 
 template <class V, bool EVEN, class T, class NB>
-__device__ inline void aa_store_raw(T* __restrict__ f, long long stride, const NB& nb,
-                                    long long cell, const T (&v)[V::q]) {
+__device__ inline void aa_store_raw(void* const* planes, const NB& nb, uint32_t cell,
+                                    const T (&v)[V::q]) {
 #pragma unroll
     for (int i = 0; i < V::q; ++i) {
         if (EVEN)
-            f[i * stride + nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))] = v[V::opp(i)];
+            elem<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))) = v[V::opp(i)];
         else
-            f[i * stride + cell] = v[i];
+            elem<T>(planes[i], cell) = v[i];
     }
 }
 
@@ -275,22 +284,6 @@ __device__ inline float ref_vv(const float (&u)[3]) {
     return vv;
 }
 
-// ---- float arithmetic (kF32F, on populations) ------------------------------------------
-template <class V>
-__device__ inline void f32_moments(const float (&f)[V::q], float& r, float (&u)[3]) {
-    r = 0.f;
-    float j[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        r += f[i];
-#pragma unroll
-        for (int a = 0; a < V::d; ++a)
-            if (V::c(i, a) != 0) j[a] += V::c(i, a) > 0 ? f[i] : -f[i];
-    }
-#pragma unroll
-    for (int a = 0; a < V::d; ++a) u[a] = j[a] / r;
-}
-
 template <class V>
 __device__ inline float cdotu(int i, const float (&u)[3]) {
     float cu = 0.f;
@@ -300,38 +293,7 @@ __device__ inline float cdotu(int i, const float (&u)[3]) {
     return cu;
 }
 
-template <class V>
-__device__ inline void f32_bgk(float (&f)[V::q], const float (&u)[3], float vv, float r,
-                               float om) {
-    const float base = 1.0f - 1.5f * vv;
-#pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        const float cu = cdotu<V>(i, u);
-        const float feq = (V::w(i) * r) * (base + cu * (3.0f + 4.5f * cu));
-        f[i] += om * (feq - f[i]);
-    }
-}
-
-template <class V>
-__device__ inline void f32_trt(float (&f)[V::q], const float (&u)[3], float vv, float r,
-                               float omp, float omn) {
-    const float base = 1.0f - 1.5f * vv;
-    f[0] += omp * ((V::w(0) * r) * base - f[0]);
-#pragma unroll
-    for (int i = 1; i < V::q; i += 2) {
-        const int j = i + 1;
-        const float cu = cdotu<V>(i, u);
-        const float wr = V::w(i) * r;
-        const float eqp = wr * (base + 4.5f * cu * cu);
-        const float eqm = wr * (3.0f * cu);
-        const float dp = omp * (eqp - 0.5f * (f[i] + f[j]));
-        const float dm = omn * (eqm - 0.5f * (f[i] - f[j]));
-        f[i] += dp + dm;
-        f[j] += dp - dm;
-    }
-}
-
-// ---- float arithmetic on deviations d_i = f_i - w_i (shifted storage) -------------------
+// ---- float arithmetic on deviations d_i = f_i - w_i (kF32F and the shifted formats) ----
 // rho - 1 = sum d, j = sum c d (the w_i cancel pairwise), and the update
 //   d_i += omega (w_i (rho - 1) + w_i rho (3 cu + 9/2 cu^2 - 3/2 u.u) - d_i)
 // never forms w_i + d_i, which keeps the small deviations exact to float rounding.
@@ -346,9 +308,9 @@ __device__ inline void dev_moments(const float (&d)[V::q], float& dr, float (&u)
         for (int a = 0; a < V::d; ++a)
             if (V::c(i, a) != 0) j[a] += V::c(i, a) > 0 ? d[i] : -d[i];
     }
-    const float r = 1.0f + dr;
+    const float inv = 1.0f / (1.0f + dr);
 #pragma unroll
-    for (int a = 0; a < V::d; ++a) u[a] = j[a] / r;
+    for (int a = 0; a < V::d; ++a) u[a] = j[a] * inv;
 }
 
 template <class V>
@@ -480,7 +442,7 @@ __device__ inline void ref_ad(float (&g)[V::q], float& r, const float (&u)[3], b
 
 // Maps the work-item to its cell.  Returns false for work-items past the end of a row.
 template <int D>
-__device__ inline bool locate(const StepParams& p, int& ix, int& iy, int& iz, long long& cell) {
+__device__ inline bool locate(const StepParams& p, int& ix, int& iy, int& iz, uint32_t& cell) {
     const int b = xcd_swizzle((int)blockIdx.x, p.nblocks);
     const int row = b / p.cpr;
     const int chunk = b - row * p.cpr;
@@ -495,83 +457,92 @@ __device__ inline bool locate(const StepParams& p, int& ix, int& iy, int& iz, lo
         iy = row - ix * p.n1;
         iz = zc;
     }
-    cell = (long long)row * p.len + zc;
+    cell = (uint32_t)row * (uint32_t)p.len + (uint32_t)zc;
     return true;
 }
 
 // ---------------------------------------------------------------------------------------
 // The fluid step, optionally fused with one advection-diffusion tracer (TQ = 5 or 7).
 // ---------------------------------------------------------------------------------------
+// Waves per SIMD the register allocation must allow: more resident waves keep more
+// loads in flight, which is what bounds this memory-streaming kernel.
+template <int Q, int S, int TQ>
+constexpr int waves_target() {
+#ifdef BOLT_WAVES
+    return BOLT_WAVES;
+#else
+    if (S == kF64 || TQ != 0) return 2;
+    return Q == 27 ? 4 : 8;
+#endif
+}
+
 template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
-__global__ void __launch_bounds__(256) lbm_step(const StepParams p) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(waves_target<Q, S, TQ>())))
+    lbm_step(const StepParams p) {
     using V = VSet<D, Q>;
     using ST = Store<S>;
     using T = typename ST::T;
     int ix, iy, iz;
-    long long cell;
+    uint32_t cell;
     if (!locate<D>(p, ix, iy, iz, cell)) return;
-    const int flag = p.flags[cell];
+    const int flag = elem<uint8_t>((const void*)p.flags, cell);
     if (flag & 1) return;
     const bool fixed = (flag & 6) != 0;
-    Nbr<D> nb = [&]() {
-        if constexpr (D == 2) return Nbr<2>(ix, iy, p);
-        else return Nbr<3>(ix, iy, iz, p);
-    }();
-    T* __restrict__ fp = (T*)p.f;
+    const Nbr<D> nb(ix, iy, iz, p);
 
     if constexpr (S == kF64) {
         double f[Q];
-        aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, f);
-        const double* rho = (const double*)p.rho;
-        const double* vel = (const double*)p.vel;
+        aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
         double rfix = 0.0, ufix[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
         if (fixed) {
-            rfix = rho[cell];
+            rfix = elem<double>((const void*)p.rho, cell);
 #pragma unroll
-            for (int a = 0; a < D; ++a) ufix[a] = vel[a * p.ncell + cell];
+            for (int a = 0; a < D; ++a) ufix[a] = elem<double>((const void*)p.vel[a], cell);
         }
         const bool use_acc = p.acc_mode != kAccNone;
         if (p.acc_mode == kAccConst) {
 #pragma unroll
             for (int a = 0; a < D; ++a) acc[a] = (double)p.g[a];
         } else if (p.acc_mode == kAccBoussinesq) {
-            const double b = (double)p.alpha * (((const double*)p.bq_C)[cell] - (double)p.c0);
+            const double b = (double)p.alpha * (elem<double>(p.bq_C, cell) - (double)p.c0);
 #pragma unroll
             for (int a = 0; a < D; ++a) acc[a] = (double)p.g[a] * b;
         }
         double r, u[3];
         f64_step<V>(f, r, u, fixed, COLL, p, rfix, ufix, acc, use_acc);
-        aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+        aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
         if (p.write_macro && !fixed) {
-            ((double*)p.rho)[cell] = r;
+            elem<double>(p.rho, cell) = r;
 #pragma unroll
-            for (int a = 0; a < D; ++a) ((double*)p.vel)[a * p.ncell + cell] = u[a];
+            for (int a = 0; a < D; ++a) elem<double>(p.vel[a], cell) = u[a];
         }
         if constexpr (TQ != 0) {
             using W = VSet<D, TQ>;
-            double* gp = (double*)p.tg;
             double g[TQ];
-            aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
             const bool tfixed = (flag & 8) != 0;
-            double* C = (double*)p.tC;
             double rt;
-            f64_ad<W>(g, rt, u, tfixed, (double)p.t_om, tfixed ? C[cell] : 0.0);
-            aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
-            C[cell] = rt;
+            f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
+                      tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
+            aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+            elem<double>(p.tC, cell) = rt;
         }
     } else {
-        float* rho = (float*)p.rho;
-        float* vel = (float*)p.vel;
         float f[Q];
         float r, u[3] = {0.f, 0.f, 0.f};
         float omp = p.om_p, omn = p.om_n;
-        if constexpr (S == kF32 || S == kF32F) {
-            aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, f);
-            if constexpr (S == kF32) ref_moments<V>(f, r, u);
-            else f32_moments<V>(f, r, u);
+        if constexpr (S == kF32) {
+            aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
+            ref_moments<V>(f, r, u);
+        } else if constexpr (S == kF32F) {
+            aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] -= V::w(i);
+            dev_moments<V>(f, r, u);  // r holds rho - 1 here
         } else {
             T raw[Q];
-            aa_load_raw<V, EVEN>(fp, p.stride, nb, cell, raw);
+            aa_load_raw<V, EVEN>(p.fpl, nb, cell, raw);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 if constexpr (S == kF16S) f[i] = (float)raw[i];
@@ -585,7 +556,7 @@ __global__ void __launch_bounds__(256) lbm_step(const StepParams p) {
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] = p.g[a];
             } else {
-                const float b = p.alpha * (((const float*)p.bq_C)[cell] - p.c0);
+                const float b = p.alpha * (elem<float>(p.bq_C, cell) - p.c0);
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] = p.g[a] * b;
             }
@@ -595,21 +566,24 @@ __global__ void __launch_bounds__(256) lbm_step(const StepParams p) {
         if (fixed) {
             omp = 1.0f;
             omn = 1.0f;
-            r = rho[cell];
-            if constexpr (S == kF16S || S == kBF16S) r = r - 1.0f;
+            r = elem<float>((const void*)p.rho, cell);
+            if constexpr (S != kF32) r = r - 1.0f;
 #pragma unroll
-            for (int a = 0; a < D; ++a) u[a] = vel[a * p.ncell + cell];
+            for (int a = 0; a < D; ++a) u[a] = elem<float>((const void*)p.vel[a], cell);
         }
         if constexpr (S == kF32) {
             const float vv = ref_vv<D>(u);
             if constexpr (COLL == 0) ref_bgk<V>(f, u, vv, r, omp);
             else ref_trt<V>(f, u, vv, r, omp, omn);
-            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+            aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
         } else if constexpr (S == kF32F) {
             const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            if constexpr (COLL == 0) f32_bgk<V>(f, u, vv, r, omp);
-            else f32_trt<V>(f, u, vv, r, omp, omn);
-            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, f);
+            if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
+            else dev_trt<V>(f, u, vv, r, omp, omn);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] += V::w(i);
+            aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
+            r = r + 1.0f;
         } else {
             const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
             if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
@@ -620,26 +594,25 @@ __global__ void __launch_bounds__(256) lbm_step(const StepParams p) {
                 if constexpr (S == kF16S) raw[i] = (_Float16)f[i];
                 else raw[i] = f32_to_bf16_rne(f[i]);
             }
-            aa_store_raw<V, EVEN>(fp, p.stride, nb, cell, raw);
+            aa_store_raw<V, EVEN>(p.fpl, nb, cell, raw);
             r = r + 1.0f;
         }
         if (p.write_macro && !fixed) {
-            rho[cell] = r;
+            elem<float>(p.rho, cell) = r;
 #pragma unroll
-            for (int a = 0; a < D; ++a) vel[a * p.ncell + cell] = u[a];
+            for (int a = 0; a < D; ++a) elem<float>(p.vel[a], cell) = u[a];
         }
         if constexpr (TQ != 0) {
             static_assert(S == kF32, "fused tracers use float storage");
             using W = VSet<D, TQ>;
-            float* gp = (float*)p.tg;
             float g[TQ];
-            aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+            aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
             const bool tfixed = (flag & 8) != 0;
-            float* C = (float*)p.tC;
             float rt;
-            ref_ad<W>(g, rt, u, tfixed, p.t_om, tfixed ? C[cell] : 0.0f);
-            aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
-            C[cell] = rt;
+            ref_ad<W>(g, rt, u, tfixed, p.t_om,
+                      tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
+            aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+            elem<float>(p.tC, cell) = rt;
         }
     }
 }
@@ -649,39 +622,34 @@ template <int D, int TQ, int S, bool EVEN>
 __global__ void __launch_bounds__(256) ad_step(const StepParams p) {
     using W = VSet<D, TQ>;
     int ix, iy, iz;
-    long long cell;
+    uint32_t cell;
     if (!locate<D>(p, ix, iy, iz, cell)) return;
-    const int flag = p.flags[cell];
+    const int flag = elem<uint8_t>((const void*)p.flags, cell);
     if (flag & 1) return;
     const bool tfixed = (flag & 8) != 0;
-    Nbr<D> nb = [&]() {
-        if constexpr (D == 2) return Nbr<2>(ix, iy, p);
-        else return Nbr<3>(ix, iy, iz, p);
-    }();
+    const Nbr<D> nb(ix, iy, iz, p);
     if constexpr (S == kF64) {
-        double* gp = (double*)p.tg;
         double g[TQ];
-        aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
         double u[3] = {0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < D; ++a) u[a] = ((const double*)p.vel)[a * p.ncell + cell];
-        double* C = (double*)p.tC;
+        for (int a = 0; a < D; ++a) u[a] = elem<double>((const void*)p.vel[a], cell);
         double rt;
-        f64_ad<W>(g, rt, u, tfixed, (double)p.t_om, tfixed ? C[cell] : 0.0);
-        aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
-        C[cell] = rt;
+        f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
+                  tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
+        aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+        elem<double>(p.tC, cell) = rt;
     } else {
-        float* gp = (float*)p.tg;
         float g[TQ];
-        aa_load_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
+        aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
         float u[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int a = 0; a < D; ++a) u[a] = ((const float*)p.vel)[a * p.ncell + cell];
-        float* C = (float*)p.tC;
+        for (int a = 0; a < D; ++a) u[a] = elem<float>((const void*)p.vel[a], cell);
         float rt;
-        ref_ad<W>(g, rt, u, tfixed, p.t_om, tfixed ? C[cell] : 0.0f);
-        aa_store_raw<W, EVEN>(gp, p.tstride, nb, cell, g);
-        C[cell] = rt;
+        ref_ad<W>(g, rt, u, tfixed, p.t_om,
+                  tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
+        aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+        elem<float>(p.tC, cell) = rt;
     }
 }
 
