@@ -105,6 +105,24 @@ def cpu_baseline(budget_s: float = 12.0):
                       f"{steps} steps, {dt:.1f} s"}
 
 
+def copy_calibration(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of nbytes (hipMemcpy): a kernel of known traffic for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy calibration: 3 x {nbytes} bytes device-to-device", file=sys.stderr,
+          flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +131,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -121,6 +140,8 @@ def main() -> int:
         return 2
     steps = args.steps + args.steps % 2
     n = args.n
+    if args.copy_calibration:
+        copy_calibration()
     rho, vel = tgv(n)
     print(f"[bench] TGV D3Q27 {n}^3: building inputs done", file=sys.stderr, flush=True)
 
