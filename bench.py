@@ -34,6 +34,19 @@ B_ALG = {"f32": 2 * 27 * 4 + 1, "f32f": 2 * 27 * 4 + 1, "f16s": 2 * 27 * 2 + 1,
          "bf16s": 2 * 27 * 2 + 1, "f64": 2 * 27 * 8 + 1}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the FP32 D3Q27 update from the committed counter summary
+    (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, calibrated on a 1 GiB copy)."""
+    path = os.path.join(ROOT, "profiles", "lbm_step_d3q27_f32_tgv256_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def tgv(n: int, u0: float = 0.04):
     k = 2 * np.pi / n
     i = np.arange(n, dtype=np.float64)
@@ -92,9 +105,10 @@ def cpu_baseline(budget_s: float = 12.0):
     sim = oracle.OracleSimulation((n, n, n), om)
     sim.rho[...] = rho
     sim.vel[...] = vel
+    sim.iterate(2)  # first call: equilibrium initialisation
     t0 = time.perf_counter()
-    sim.iterate(2)
-    per = (time.perf_counter() - t0) / 2
+    sim.iterate(4)
+    per = (time.perf_counter() - t0) / 4
     steps = max(2, int(budget_s / max(per, 1e-6)) // 2 * 2)
     t0 = time.perf_counter()
     sim.iterate(steps)
@@ -103,6 +117,29 @@ def cpu_baseline(budget_s: float = 12.0):
             "kind": "port",
             "sample": f"oracle (OpenMP C++ restatement of lib.cl) D3Q27 TRT TGV {n}^3, "
                       f"{steps} steps, {dt:.1f} s"}
+
+
+def rb_batch(steps: int = 50000):
+    """64 Rayleigh-Benard cells of uneven sizes (D2Q9 + D2Q5 + Boussinesq), one launch per
+    step over all of them, graph-replayed, shape picked by the autotuner."""
+    from boltzmann_amd import rb
+    specs = rb.sweep(64, seed=4)
+    b = rb.make_batch(specs)
+    report = b.tune(40)
+    b.step(20)
+    ms = b.step_timed(steps)
+    cells = b.cells
+    mlups = cells * steps / (1e6 * ms * 1e-3)
+    nus = []
+    for k in (0, 1, 2, 3):
+        T = b.read(k, "C")
+        uy = b.read(k, "vel")[..., 1]
+        nus.append(round(rb.nusselt(T, uy, specs[k]), 3))
+    return {"instances": len(specs), "cells": int(cells), "steps": steps,
+            "mlups": round(mlups, 1), "ms_per_step": round(ms / steps, 5),
+            "hbm_frac": round(mlups * 1e6 * 121 / (PEAK_HBM_GBS * 1e9), 4),
+            "shape": report.strip().splitlines()[-1],
+            "nusselt_first4": nus, "ra_first4": [round(s.ra) for s in specs[:4]]}
 
 
 def copy_calibration(nbytes: int = 1 << 30) -> None:
@@ -132,6 +169,7 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--rb-steps", type=int, default=50000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -162,6 +200,9 @@ def main() -> int:
             variants[f"{coll}_{st}"] = v
             print(f"[bench] {coll} {st}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["trt_f64"] = {"mlups": round(ref64["mlups"], 1)}
+        variants["rb_batch"] = rb_batch(args.rb_steps)
+        print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
+              flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
 
     achieved = head["mlups"] * 1e6 * B_ALG["f32"] / 1e9  # GB/s
@@ -184,7 +225,8 @@ def main() -> int:
                    "storage": "f32"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None, "bytes_per_lup": B_ALG["f32"]},
+                     "traffic": measured_traffic() if n == 256 else None,
+                     "bytes_per_lup": B_ALG["f32"]},
         "wall_mlups": round(head["wall_mlups"], 1),
         "mass_drift": head["mass_drift"],
         "variants": variants,

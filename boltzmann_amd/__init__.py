@@ -11,6 +11,7 @@ Importing fails loudly if the native module has not been built
 
 try:
     from ._boltzmann import (  # noqa: F401
+        Batch,
         BoltError,
         Cells,
         Fluid,
@@ -25,4 +26,4 @@ except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
         f"(HIP/gfx950 toolchain required): {exc}"
     ) from exc
 
-__all__ = ["BoltError", "Cells", "Fluid", "Omega", "Scalar", "Simulation", "version"]
+__all__ = ["Batch", "BoltError", "Cells", "Fluid", "Omega", "Scalar", "Simulation", "version"]

@@ -227,6 +227,29 @@ struct PySimulation {
     std::shared_ptr<Core> core;
 };
 
+struct BatchCore {
+    bolt_ctx* ctx = nullptr;
+    bolt_batch* batch = nullptr;
+    std::vector<std::pair<int64_t, int64_t>> shapes;
+    ~BatchCore() {
+        if (batch) bolt_batch_destroy(batch);
+        if (ctx) bolt_ctx_destroy(ctx);
+    }
+};
+struct PyBatch {
+    std::shared_ptr<BatchCore> core;
+};
+
+std::pair<int, int> batch_field(const std::string& f) {
+    if (f == "flags") return {BOLT_FIELD_FLAGS, 1};
+    if (f == "f") return {BOLT_FIELD_F, 9};
+    if (f == "rho") return {BOLT_FIELD_RHO, 1};
+    if (f == "vel") return {BOLT_FIELD_VEL, 2};
+    if (f == "g") return {BOLT_FIELD_G, 5};
+    if (f == "C") return {BOLT_FIELD_C, 1};
+    throw BoltError("unknown field " + f);
+}
+
 size_t fluid_size_bytes(const Core& c) {
     return 4 * (c.f.size() + c.rho.size() + c.vel.size() +
                 ((c.has_gravity || !c.couplings.empty()) ? c.ncell * c.d : 0));
@@ -524,6 +547,101 @@ PYBIND11_MODULE(_boltzmann, m) {
             check(bolt_lattice_device_bytes(s.core->lat, &b), "device_bytes");
             return b;
         });
+
+    // ---- batches of small coupled lattices (no counterpart in the reference API) ----
+    py::class_<PyBatch>(m, "Batch")
+        .def(py::init([](const std::string& dev, std::vector<std::vector<int64_t>> counts,
+                         const std::string& collision, std::vector<std::vector<float>> omega_f,
+                         std::vector<float> omega_g, std::vector<float> alpha,
+                         std::vector<float> c0, std::vector<std::vector<float>> g) {
+                 const size_t n = counts.size();
+                 if (omega_f.size() != n || omega_g.size() != n || alpha.size() != n ||
+                     c0.size() != n || g.size() != n)
+                     throw BoltError("Batch: every per-instance list needs one entry per instance");
+                 std::vector<int64_t> cc;
+                 std::vector<float> of, gg;
+                 std::vector<std::pair<int64_t, int64_t>> shapes;
+                 for (size_t k = 0; k < n; ++k) {
+                     if (counts[k].size() != 2 || g[k].size() != 2)
+                         throw BoltError("Batch: instances are 2D (counts and g of length 2)");
+                     cc.push_back(counts[k][0]);
+                     cc.push_back(counts[k][1]);
+                     shapes.push_back({counts[k][0], counts[k][1]});
+                     of.push_back(omega_f[k].at(0));
+                     of.push_back(omega_f[k].size() > 1 ? omega_f[k][1] : omega_f[k][0]);
+                     gg.push_back(g[k][0]);
+                     gg.push_back(g[k][1]);
+                 }
+                 int coll;
+                 if (collision == "bgk") coll = BOLT_BGK;
+                 else if (collision == "trt") coll = BOLT_TRT;
+                 else throw BoltError("collision must be 'bgk' or 'trt'");
+                 auto b = std::make_shared<BatchCore>();
+                 check(bolt_ctx_create(device_kind(dev), 0, &b->ctx), "device");
+                 check(bolt_batch_create(b->ctx, (int)n, cc.data(), coll, of.data(), omega_g.data(),
+                                         alpha.data(), c0.data(), gg.data(), &b->batch), "Batch");
+                 b->shapes = shapes;
+                 return PyBatch{b};
+             }),
+             py::arg("dev"), py::arg("counts"), py::arg("collision"), py::arg("omega_f"),
+             py::arg("omega_g"), py::arg("alpha"), py::arg("c0"), py::arg("g"))
+        .def_property_readonly("n", [](PyBatch& b) { return b.core->shapes.size(); })
+        .def_property_readonly("cells", [](PyBatch& b) {
+            uint64_t c = 0;
+            check(bolt_batch_cells(b.core->batch, &c), "cells");
+            return c;
+        })
+        .def("write", [](PyBatch& b, int k, const std::string& field, py::array arr) {
+            auto [fid, per] = batch_field(field);
+            const auto& sh = b.core->shapes.at(k);
+            const size_t n = (size_t)sh.first * sh.second * per;
+            if (fid == BOLT_FIELD_FLAGS) {
+                auto a = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(arr);
+                if ((size_t)a.size() != n) throw BoltError("write: wrong size");
+                check(bolt_batch_write(b.core->batch, k, fid, BOLT_DT_I32, a.data(), n * 4), "write");
+            } else {
+                auto a = py::array_t<float, py::array::c_style | py::array::forcecast>(arr);
+                if ((size_t)a.size() != n) throw BoltError("write: wrong size");
+                check(bolt_batch_write(b.core->batch, k, fid, BOLT_DT_F32, a.data(), n * 4), "write");
+            }
+        }, py::arg("inst"), py::arg("field"), py::arg("array"))
+        .def("read", [](PyBatch& b, int k, const std::string& field) {
+            auto [fid, per] = batch_field(field);
+            const auto& sh = b.core->shapes.at(k);
+            std::vector<ssize_t> shp = {(ssize_t)sh.first, (ssize_t)sh.second};
+            if (per > 1 || field == "vel") shp.push_back(per);
+            const size_t n = (size_t)sh.first * sh.second * per;
+            if (fid == BOLT_FIELD_FLAGS) {
+                py::array_t<int32_t> out(shp);
+                check(bolt_batch_read(b.core->batch, k, fid, BOLT_DT_I32, out.mutable_data(), n * 4), "read");
+                return py::array(out);
+            }
+            py::array_t<float> out(shp);
+            check(bolt_batch_read(b.core->batch, k, fid, BOLT_DT_F32, out.mutable_data(), n * 4), "read");
+            return py::array(out);
+        }, py::arg("inst"), py::arg("field"))
+        .def("equilibrate", [](PyBatch& b) { check(bolt_batch_equilibrate(b.core->batch), "equilibrate"); })
+        .def("step", [](PyBatch& b, uint64_t n) { check(bolt_batch_step(b.core->batch, n), "step"); }, py::arg("iters"))
+        .def("step_timed", [](PyBatch& b, uint64_t n) {
+            double ms = 0;
+            check(bolt_batch_step_timed(b.core->batch, n, &ms), "step_timed");
+            return ms;
+        }, py::arg("iters"))
+        .def("set_shape", [](PyBatch& b, int block, int graph_steps) {
+            check(bolt_batch_set_shape(b.core->batch, block, graph_steps), "set_shape");
+        }, py::arg("block"), py::arg("graph_steps"))
+        .def("tune", [](PyBatch& b, int steps) {
+            char rep[2048];
+            check(bolt_batch_tune(b.core->batch, steps, rep, sizeof rep), "tune");
+            return std::string(rep);
+        }, py::arg("steps") = 40)
+        .def("reduce", [](PyBatch& b, int k, const std::string& what) {
+            double out[3] = {0, 0, 0};
+            int w = what == "mass" ? BOLT_REDUCE_MASS : what == "tracer" ? BOLT_REDUCE_TRACER : -1;
+            if (w < 0) throw BoltError("reduce: 'mass' or 'tracer'");
+            check(bolt_batch_reduce(b.core->batch, k, w, out), "reduce");
+            return out[0];
+        }, py::arg("inst"), py::arg("what"));
 
     m.def("version", []() { return std::string(bolt_version()); });
 }

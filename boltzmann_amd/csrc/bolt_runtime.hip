@@ -18,6 +18,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/bolt.h"
 #include "update.hpp"
@@ -946,4 +947,392 @@ int bolt_lattice_set_block(bolt_lattice* L, int block) {
         return fail(BOLT_EINVAL, "block must be 64, 128, 192 or 256 (or 0)");
     L->block = block;
     return BOLT_OK;
+}
+
+// =======================================================================================
+// Batches: many independent small D2Q9 + D2Q5 Boussinesq lattices advanced together.
+//
+// Each instance is an ordinary bolt_lattice (one fluid, one tracer, one coupling), so
+// transfers, the equilibrium initialisation and the reductions are the lattice ones.  A
+// step is ONE launch over all instances: every work-group reads its tile (instance, row,
+// chunk of the contiguous axis) and the instance's descriptor, and runs the same
+// update_cell as the single-lattice kernel, so every instance evolves bit-identically to
+// running it alone.  Steps are captured into a HIP graph and replayed; kernel boundaries
+// order the in-place AA steps.  bolt_batch_tune times the tile width and the graph length
+// on the batch itself and keeps the fastest.
+// =======================================================================================
+struct InstDesc {
+    char* f;            // plane 0 of the fluid populations
+    char* g;            // plane 0 of the tracer populations
+    void* rho;
+    void* vel0;
+    void* vel1;
+    void* C;
+    const uint8_t* flags;
+    uint32_t plane_bytes;
+    int nx, ny;
+    int acc_mode;
+    float om_p, om_n, om_g;
+    float alpha, c0, gx, gy;
+};
+
+struct Tile {
+    uint16_t inst, row, chunk, pad;
+};
+
+struct BatchParams {
+    const InstDesc* inst;
+    const Tile* tiles;
+    int ntiles;
+    int write_macro;
+};
+
+template <int COLL, bool EVEN>
+__global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
+    const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
+    const Tile t = bp.tiles[b];
+    const InstDesc& d = bp.inst[t.inst];
+    const int iy = (int)t.chunk * (int)blockDim.x + (int)threadIdx.x;
+    if (iy >= d.ny) return;
+    StepParams p;
+    p.n0 = d.nx;
+    p.n1 = d.ny;
+    p.n2 = 1;
+    p.len = d.ny;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p.fpl[i] = d.f + (size_t)i * d.plane_bytes;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) p.gpl[i] = d.g + (size_t)i * d.plane_bytes;
+    p.rho = d.rho;
+    p.vel[0] = d.vel0;
+    p.vel[1] = d.vel1;
+    p.flags = d.flags;
+    p.om_p = d.om_p;
+    p.om_n = d.om_n;
+    p.acc_mode = d.acc_mode;
+    p.g[0] = d.gx;
+    p.g[1] = d.gy;
+    p.alpha = d.alpha;
+    p.c0 = d.c0;
+    p.bq_C = d.C;
+    p.write_macro = bp.write_macro;
+    p.tC = d.C;
+    p.t_om = d.om_g;
+    update_cell<2, 9, COLL, kF32, EVEN, 5>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
+}
+
+struct bolt_batch {
+    bolt_ctx* ctx = nullptr;
+    int collision = 0;
+    std::vector<bolt_lattice*> lat;
+    InstDesc* d_inst = nullptr;
+    Tile* d_tiles = nullptr;
+    int ntiles = 0;
+    int block = 128;     // tile width (cells of the contiguous axis per work-group)
+    int graph_steps = 100;
+    hipGraphExec_t graph = nullptr;
+    int graph_block = 0, graph_len = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t cells = 0;
+};
+
+static void batch_drop_graph(bolt_batch* B) {
+    if (B->graph) hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+    B->graph_block = B->graph_len = 0;
+}
+
+static int batch_build_tiles(bolt_batch* B, int block) {
+    std::vector<Tile> tiles;
+    for (size_t k = 0; k < B->lat.size(); ++k) {
+        const bolt_lattice* L = B->lat[k];
+        const int cpr = (L->n[1] + block - 1) / block;
+        for (int r = 0; r < L->n[0]; ++r)
+            for (int c = 0; c < cpr; ++c) tiles.push_back({(uint16_t)k, (uint16_t)r, (uint16_t)c, 0});
+    }
+    hipFree(B->d_tiles);
+    B->d_tiles = nullptr;
+    HIP_TRY(hipMalloc(&B->d_tiles, tiles.size() * sizeof(Tile)));
+    HIP_TRY(hipMemcpy(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    B->ntiles = (int)tiles.size();
+    B->block = block;
+    batch_drop_graph(B);
+    return BOLT_OK;
+}
+
+static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
+    BatchParams bp{B->d_inst, B->d_tiles, B->ntiles, write_macro ? 1 : 0};
+    const dim3 grid((unsigned)B->ntiles), blk((unsigned)B->block);
+    if (B->collision == BOLT_BGK) {
+        if (even) hipLaunchKernelGGL((rb_batch_step<0, true>), grid, blk, 0, s, bp);
+        else hipLaunchKernelGGL((rb_batch_step<0, false>), grid, blk, 0, s, bp);
+    } else {
+        if (even) hipLaunchKernelGGL((rb_batch_step<1, true>), grid, blk, 0, s, bp);
+        else hipLaunchKernelGGL((rb_batch_step<1, false>), grid, blk, 0, s, bp);
+    }
+}
+
+static int batch_capture(bolt_batch* B, int len) {
+    batch_drop_graph(B);
+    hipStream_t s = B->ctx->stream;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < len; ++k) batch_launch(B, (k % 2) == 0, false, s);
+    HIP_TRY(hipStreamEndCapture(s, &g));
+    hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(BOLT_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    B->graph_block = B->block;
+    B->graph_len = len;
+    return BOLT_OK;
+}
+
+static int batch_enqueue(bolt_batch* B, uint64_t n) {
+    if (n % 2 != 0) return fail(BOLT_EINVAL, "iters must be even");
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    hipStream_t s = B->ctx->stream;
+    const uint64_t len = (uint64_t)B->graph_steps;
+    uint64_t done = 0;
+    if (n > len + 2) {
+        if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len) {
+            int rc = batch_capture(B, (int)len);
+            if (rc) return rc;
+        }
+        const uint64_t replays = (n - 2) / len;
+        for (uint64_t r = 0; r < replays; ++r) HIP_TRY(hipGraphLaunch(B->graph, s));
+        done = replays * len;
+    }
+    for (uint64_t k = done; k < n; ++k) batch_launch(B, (k % 2) == 0, k + 1 == n, s);
+    HIP_TRY(hipGetLastError());
+    return BOLT_OK;
+}
+
+static int batch_upload_desc(bolt_batch* B) {
+    std::vector<InstDesc> h(B->lat.size());
+    for (size_t k = 0; k < B->lat.size(); ++k) {
+        bolt_lattice* L = B->lat[k];
+        InstDesc& d = h[k];
+        d.f = (char*)L->f;
+        d.g = (char*)L->tracers[0].g;
+        d.rho = L->rho;
+        d.vel0 = L->vel;
+        d.vel1 = (char*)L->vel + (size_t)L->ncell * 4;
+        d.C = L->tracers[0].C;
+        d.flags = L->flags;
+        d.plane_bytes = (uint32_t)(L->stride * 4);
+        d.nx = L->n[0];
+        d.ny = L->n[1];
+        d.om_p = L->om_p;
+        d.om_n = L->om_n;
+        d.om_g = L->tracers[0].omega;
+        if (!L->couplings.empty()) {
+            const Coupling& c = L->couplings.back();
+            d.acc_mode = kAccBoussinesq;
+            d.alpha = c.alpha;
+            d.c0 = c.c0;
+            d.gx = c.g[0];
+            d.gy = c.g[1];
+        } else {
+            d.acc_mode = L->has_gravity ? kAccConst : kAccNone;
+            d.gx = L->gravity[0];
+            d.gy = L->gravity[1];
+        }
+    }
+    hipFree(B->d_inst);
+    B->d_inst = nullptr;
+    HIP_TRY(hipMalloc(&B->d_inst, h.size() * sizeof(InstDesc)));
+    HIP_TRY(hipMemcpy(B->d_inst, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
+    return BOLT_OK;
+}
+
+int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision,
+                      const float* omega_f, const float* omega_g, const float* alpha,
+                      const float* c0, const float* g, bolt_batch** out) {
+    if (!ctx || !counts || !omega_f || !omega_g || !alpha || !c0 || !g || !out)
+        return fail(BOLT_EINVAL, "null argument");
+    *out = nullptr;
+    if (n < 1 || n > 65535) return fail(BOLT_EINVAL, "instance count %d out of range", n);
+    auto B = std::make_unique<bolt_batch>();
+    B->ctx = ctx;
+    B->collision = collision;
+    for (int k = 0; k < n; ++k) {
+        if (counts[2 * k] > 65535) {
+            for (auto* L : B->lat) bolt_lattice_destroy(L);
+            return fail(BOLT_EINVAL, "instance %d: nx above 65535", k);
+        }
+        bolt_lattice* L = nullptr;
+        int rc = bolt_lattice_create(ctx, 2, counts + 2 * k, 9, collision, omega_f[2 * k],
+                                     omega_f[2 * k + 1], BOLT_STORE_F32, &L);
+        int tid = 0;
+        if (!rc) B->lat.push_back(L);
+        if (!rc) rc = bolt_lattice_add_tracer(L, 5, omega_g[k], &tid);
+        if (!rc) rc = bolt_lattice_add_coupling(L, tid, alpha[k], c0[k], g + 2 * k);
+        if (rc) {
+            std::string msg = std::string("instance ") + std::to_string(k) + ": " + g_last_error;
+            for (auto* l : B->lat) bolt_lattice_destroy(l);
+            return fail(rc, "%s", msg.c_str());
+        }
+        B->cells += (uint64_t)L->ncell;
+    }
+    HIP_TRY(hipEventCreate(&B->ev0));
+    HIP_TRY(hipEventCreate(&B->ev1));
+    int rc = batch_upload_desc(B.get());
+    if (!rc) rc = batch_build_tiles(B.get(), 128);
+    if (rc) {
+        for (auto* l : B->lat) bolt_lattice_destroy(l);
+        return rc;
+    }
+    *out = B.release();
+    return BOLT_OK;
+}
+
+int bolt_batch_destroy(bolt_batch* B) {
+    if (!B) return BOLT_OK;
+    hipSetDevice(B->ctx->device);
+    hipStreamSynchronize(B->ctx->stream);
+    batch_drop_graph(B);
+    for (auto* L : B->lat) bolt_lattice_destroy(L);
+    hipFree(B->d_inst);
+    hipFree(B->d_tiles);
+    if (B->ev0) hipEventDestroy(B->ev0);
+    if (B->ev1) hipEventDestroy(B->ev1);
+    delete B;
+    return BOLT_OK;
+}
+
+static int batch_lat(bolt_batch* B, int inst, bolt_lattice** L) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    if (inst < 0 || inst >= (int)B->lat.size()) return fail(BOLT_EINVAL, "unknown instance %d", inst);
+    *L = B->lat[inst];
+    return BOLT_OK;
+}
+
+int bolt_batch_write(bolt_batch* B, int inst, int field, int host_dtype, const void* host,
+                     size_t nbytes) {
+    bolt_lattice* L;
+    int rc = batch_lat(B, inst, &L);
+    return rc ? rc : bolt_lattice_write(L, field, 0, host_dtype, host, nbytes);
+}
+
+int bolt_batch_read(bolt_batch* B, int inst, int field, int host_dtype, void* host,
+                    size_t nbytes) {
+    bolt_lattice* L;
+    int rc = batch_lat(B, inst, &L);
+    return rc ? rc : bolt_lattice_read(L, field, 0, host_dtype, host, nbytes);
+}
+
+int bolt_batch_equilibrate(bolt_batch* B) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    for (auto* L : B->lat) {
+        int rc = bolt_lattice_equilibrate(L);
+        if (rc) return rc;
+    }
+    return BOLT_OK;
+}
+
+int bolt_batch_step(bolt_batch* B, uint64_t n) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    return batch_enqueue(B, n);
+}
+
+int bolt_batch_step_timed(bolt_batch* B, uint64_t n, double* ms) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    HIP_TRY(hipEventRecord(B->ev0, B->ctx->stream));
+    int rc = batch_enqueue(B, n);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(B->ev1, B->ctx->stream));
+    HIP_TRY(hipEventSynchronize(B->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, B->ev0, B->ev1));
+    if (ms) *ms = t;
+    return BOLT_OK;
+}
+
+int bolt_batch_reduce(bolt_batch* B, int inst, int what, double* out) {
+    bolt_lattice* L;
+    int rc = batch_lat(B, inst, &L);
+    return rc ? rc : bolt_lattice_reduce(L, what, 0, out);
+}
+
+int bolt_batch_cells(const bolt_batch* B, uint64_t* out) {
+    if (!B || !out) return fail(BOLT_EINVAL, "null argument");
+    *out = B->cells;
+    return BOLT_OK;
+}
+
+int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    if (block < 64 || block > 256 || block % 64 != 0)
+        return fail(BOLT_EINVAL, "block must be 64, 128, 192 or 256");
+    if (graph_steps < 2 || graph_steps % 2 != 0 || graph_steps > 4096)
+        return fail(BOLT_EINVAL, "graph_steps must be even, 2..4096");
+    B->graph_steps = graph_steps;
+    if (block != B->block) return batch_build_tiles(B, block);
+    return BOLT_OK;
+}
+
+// Autotuner: times every (tile width, graph length) candidate over `steps` steps on the
+// batch itself, restores the batch's state afterwards, and keeps the fastest shape.
+int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    if (steps < 4 || steps % 2) return fail(BOLT_EINVAL, "steps must be even and >= 4");
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    hipStream_t s = B->ctx->stream;
+    // snapshot the state the tuning steps overwrite
+    struct Saved { void* dst; void* src; size_t bytes; };
+    std::vector<Saved> saved;
+    auto keep = [&](void* p, size_t bytes) -> int {
+        void* c = nullptr;
+        HIP_TRY(hipMalloc(&c, bytes));
+        HIP_TRY(hipMemcpyAsync(c, p, bytes, hipMemcpyDeviceToDevice, s));
+        saved.push_back({p, c, bytes});
+        return BOLT_OK;
+    };
+    int rc = BOLT_OK;
+    for (auto* L : B->lat) {
+        if (!rc) rc = keep(L->f, (size_t)L->stride * 9 * 4);
+        if (!rc) rc = keep(L->tracers[0].g, (size_t)L->stride * 5 * 4);
+        if (!rc) rc = keep(L->tracers[0].C, (size_t)L->ncell * 4);
+        if (!rc) rc = keep(L->rho, (size_t)L->ncell * 4);
+        if (!rc) rc = keep(L->vel, (size_t)L->ncell * 2 * 4);
+    }
+    std::string rep;
+    int best_block = B->block, best_len = B->graph_steps;
+    double best = 1e300;
+    const int blocks[] = {64, 128, 256};
+    const int lens[] = {20, 100, 400};
+    for (int blk : blocks) {
+        if (rc) break;
+        for (int gl : lens) {
+            if (rc) break;
+            rc = bolt_batch_set_shape(B, blk, gl);
+            if (rc) break;
+            double ms = 0;
+            rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);  // warm + capture
+            if (!rc) rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);
+            if (rc) break;
+            const double per = ms / std::max(steps, gl + 4);
+            char line[96];
+            snprintf(line, sizeof line, "block %d graph %d: %.4f ms/step\n", blk, gl, per);
+            rep += line;
+            if (per < best) {
+                best = per;
+                best_block = blk;
+                best_len = gl;
+            }
+        }
+    }
+    for (auto& sv : saved) {
+        hipMemcpyAsync(sv.dst, sv.src, sv.bytes, hipMemcpyDeviceToDevice, s);
+    }
+    hipStreamSynchronize(s);
+    for (auto& sv : saved) hipFree(sv.src);
+    if (rc) return rc;
+    rc = bolt_batch_set_shape(B, best_block, best_len);
+    char line[96];
+    snprintf(line, sizeof line, "chosen: block %d graph %d\n", best_block, best_len);
+    rep += line;
+    if (report && len) snprintf(report, len, "%s", rep.c_str());
+    return rc;
 }

@@ -476,16 +476,12 @@ constexpr int waves_target() {
 #endif
 }
 
+// One cell of one time step: fluid update, optionally followed by one tracer (TQ != 0).
 template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
-__global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(waves_target<Q, S, TQ>())))
-    lbm_step(const StepParams p) {
+__device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, uint32_t cell) {
     using V = VSet<D, Q>;
     using ST = Store<S>;
     using T = typename ST::T;
-    int ix, iy, iz;
-    uint32_t cell;
-    if (!locate<D>(p, ix, iy, iz, cell)) return;
     const int flag = elem<uint8_t>((const void*)p.flags, cell);
     if (flag & 1) return;
     const bool fixed = (flag & 6) != 0;
@@ -615,6 +611,16 @@ __global__ void __launch_bounds__(256)
             elem<float>(p.tC, cell) = rt;
         }
     }
+}
+
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(waves_target<Q, S, TQ>())))
+    lbm_step(const StepParams p) {
+    int ix, iy, iz;
+    uint32_t cell;
+    if (!locate<D>(p, ix, iy, iz, cell)) return;
+    update_cell<D, Q, COLL, S, EVEN, TQ>(p, ix, iy, iz, cell);
 }
 
 // Advection-diffusion tracer on its own, reading the fluid velocity written this step.

@@ -145,6 +145,34 @@ int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);
  * 0 restores the built-in choice. */
 int bolt_lattice_set_block(bolt_lattice* lat, int block);
 
+/* ---- batches: many small 2D Rayleigh-Benard-type lattices in one launch per step -----
+ * No counterpart in the reference (it runs one Simulation per process).  Instance k is a
+ * D2Q9 fluid (counts[2k], counts[2k+1]; omega_f[2k] = w+, omega_f[2k+1] = w-, ignored for
+ * BGK) with one D2Q5 tracer (omega_g[k]) coupled through acc = g * (alpha * (C - c0))
+ * (alpha[k], c0[k], g[2k..2k+1]): the reference's set_boussinesq_acc_2d + update_d2q9_* +
+ * update_d2q5_bgk per step (simulation.rs:353-432).  Fields are the lattice ones
+ * (BOLT_FIELD_*; the tracer is implicit).  Every instance evolves exactly as it would alone. */
+typedef struct bolt_batch bolt_batch;
+int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision,
+                      const float* omega_f, const float* omega_g, const float* alpha,
+                      const float* c0, const float* g, bolt_batch** out);
+int bolt_batch_destroy(bolt_batch* b);
+int bolt_batch_write(bolt_batch* b, int inst, int field, int host_dtype, const void* host,
+                     size_t nbytes);
+int bolt_batch_read(bolt_batch* b, int inst, int field, int host_dtype, void* host,
+                    size_t nbytes);
+int bolt_batch_equilibrate(bolt_batch* b);
+/* n even; one launch per step over all instances, replayed from a captured HIP graph */
+int bolt_batch_step(bolt_batch* b, uint64_t n);
+int bolt_batch_step_timed(bolt_batch* b, uint64_t n, double* ms);
+int bolt_batch_reduce(bolt_batch* b, int inst, int what, double* out);
+int bolt_batch_cells(const bolt_batch* b, uint64_t* out);
+/* launch shape: work-group width along the contiguous axis and steps per captured graph */
+int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
+/* autotuner: times the candidate shapes on this batch (state restored afterwards), keeps
+ * the fastest and writes a per-candidate report into report[len]. */
+int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
+
 #ifdef __cplusplus
 }
 #endif
