@@ -142,6 +142,49 @@ def rb_batch(steps: int = 50000):
             "nusselt_first4": nus, "ra_first4": [round(s.ra) for s in specs[:4]]}
 
 
+def cylinder(steps_marks=(1000, 5000, 20000)):
+    """Flow past a disc in a 4096 x 1024 channel, D2Q9 TRT (magic 1/4), Re = 1000 on the
+    diameter: walls top and bottom, bounce-back disc r = 64 at (1024, 512), fixed-velocity
+    inflow u = 0.05, fixed-density outflow.  Runs f64, f32, f16s and bf16s storage and
+    reports the relative max-norm of u against the f64 run at each mark, plus MLUPS."""
+    import boltzmann_amd
+    from boltzmann_amd.core import CellFlags, trt
+    nx, ny, r, u0 = 4096, 1024, 64, 0.05
+    nu = u0 * 2 * r / 1000.0
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    disc = (X - 1024) ** 2 + (Y - 512) ** 2 < r * r
+    vel0 = np.zeros((nx, ny, 2), np.float32)
+    vel0[..., 0] = u0
+    out = {}
+    ref = {}
+    for st in ("f64", "f32", "f16s", "bf16s"):
+        sim = boltzmann_amd.Simulation("gpu", [nx, ny], trt(3 * nu + 0.5), storage=st)
+        flags = sim.cells.flags
+        flags[0, :] = CellFlags.FIXED_FLUID
+        flags[-1, :] = CellFlags.FIXED_FLUID_DENSITY
+        flags[:, 0] = CellFlags.WALL
+        flags[:, -1] = CellFlags.WALL
+        flags[disc] = CellFlags.WALL
+        v = vel0.copy()
+        v[flags == CellFlags.WALL] = 0.0
+        sim.fluid.vel[...] = v
+        done, ms_tot, errs = 0, 0.0, {}
+        sim.step(2)
+        done = 2
+        for mark in steps_marks:
+            ms_tot += sim.step_timed(mark - done)
+            done = mark
+            u = sim.read_field("vel", dtype="f64")
+            if st == "f64":
+                ref[mark] = u
+            else:
+                errs[str(mark)] = float(np.abs(u - ref[mark]).max() / np.abs(ref[mark]).max())
+        mlups = nx * ny * (done - 2) / (1e6 * ms_tot * 1e-3)
+        out[st] = {"mlups": round(mlups, 1), **({"u_err_vs_f64": errs} if errs else {})}
+        del sim
+    return out
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of nbytes (hipMemcpy): a kernel of known traffic for
     scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -200,6 +243,8 @@ def main() -> int:
             variants[f"{coll}_{st}"] = v
             print(f"[bench] {coll} {st}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["trt_f64"] = {"mlups": round(ref64["mlups"], 1)}
+        variants["cylinder_d2q9_trt_4096x1024"] = cylinder()
+        print("[bench] cylinder done", file=sys.stderr, flush=True)
         variants["rb_batch"] = rb_batch(args.rb_steps)
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)

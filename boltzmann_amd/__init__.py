@@ -15,9 +15,11 @@ try:
         BoltError,
         Cells,
         Fluid,
+        Network,
         Omega,
         Scalar,
         Simulation,
+        gemm_check,
         version,
     )
 except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
@@ -26,4 +28,4 @@ except ImportError as exc:  # pragma: no cover - exercised only on unbuilt trees
         f"(HIP/gfx950 toolchain required): {exc}"
     ) from exc
 
-__all__ = ["Batch", "BoltError", "Cells", "Fluid", "Omega", "Scalar", "Simulation", "version"]
+__all__ = ["Batch", "BoltError", "Cells", "Fluid", "Network", "Omega", "Scalar", "Simulation", "version"]

@@ -49,7 +49,7 @@ def ext_path() -> str:
 
 def build_hip() -> None:
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", os.path.join(CSRC, "bolt_runtime.hip"),
-          "-o", lib_path()])
+          os.path.join(CSRC, "bolt_nn.hip"), "-o", lib_path()])
 
 
 def build_ext() -> None:

@@ -227,6 +227,19 @@ struct PySimulation {
     std::shared_ptr<Core> core;
 };
 
+struct NetCore {
+    bolt_ctx* ctx = nullptr;
+    bolt_nn* net = nullptr;
+    int nx = 0, ny = 0, batch = 0;
+    ~NetCore() {
+        if (net) bolt_nn_destroy(net);
+        if (ctx) bolt_ctx_destroy(ctx);
+    }
+};
+struct PyNet {
+    std::shared_ptr<NetCore> core;
+};
+
 struct BatchCore {
     bolt_ctx* ctx = nullptr;
     bolt_batch* batch = nullptr;
@@ -541,6 +554,14 @@ PYBIND11_MODULE(_boltzmann, m) {
             }
             throw BoltError("unknown reduction " + what);
         }, py::arg("what"), py::arg("tracer") = py::none())
+        .def("residual", [](PySimulation& s, uint64_t steps_between) {
+            // max|u - u_prev| / (steps_between * max|u|), FP64 on the device; the first
+            // call stores the snapshot and returns inf
+            std::lock_guard<std::mutex> lk(s.core->mu);
+            double r = 0;
+            check(bolt_lattice_residual(s.core->lat, steps_between, &r), "residual");
+            return r;
+        }, py::arg("steps_between"))
         .def("set_block", [](PySimulation& s, int b) { check(bolt_lattice_set_block(s.core->lat, b), "set_block"); })
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;
@@ -552,11 +573,21 @@ PYBIND11_MODULE(_boltzmann, m) {
     py::class_<PyBatch>(m, "Batch")
         .def(py::init([](const std::string& dev, std::vector<std::vector<int64_t>> counts,
                          const std::string& collision, std::vector<std::vector<float>> omega_f,
-                         std::vector<float> omega_g, std::vector<float> alpha,
-                         std::vector<float> c0, std::vector<std::vector<float>> g) {
+                         py::object omega_g_o, py::object alpha_o, py::object c0_o,
+                         py::object g_o) {
                  const size_t n = counts.size();
-                 if (omega_f.size() != n || omega_g.size() != n || alpha.size() != n ||
-                     c0.size() != n || g.size() != n)
+                 const bool tracer = !omega_g_o.is_none();
+                 std::vector<float> omega_g, alpha, c0;
+                 std::vector<std::vector<float>> g(n, std::vector<float>{0.f, 0.f});
+                 if (tracer) {
+                     omega_g = omega_g_o.cast<std::vector<float>>();
+                     alpha = alpha_o.cast<std::vector<float>>();
+                     c0 = c0_o.cast<std::vector<float>>();
+                     g = g_o.cast<std::vector<std::vector<float>>>();
+                 }
+                 if (omega_f.size() != n ||
+                     (tracer && (omega_g.size() != n || alpha.size() != n || c0.size() != n ||
+                                 g.size() != n)))
                      throw BoltError("Batch: every per-instance list needs one entry per instance");
                  std::vector<int64_t> cc;
                  std::vector<float> of, gg;
@@ -578,13 +609,21 @@ PYBIND11_MODULE(_boltzmann, m) {
                  else throw BoltError("collision must be 'bgk' or 'trt'");
                  auto b = std::make_shared<BatchCore>();
                  check(bolt_ctx_create(device_kind(dev), 0, &b->ctx), "device");
-                 check(bolt_batch_create(b->ctx, (int)n, cc.data(), coll, of.data(), omega_g.data(),
-                                         alpha.data(), c0.data(), gg.data(), &b->batch), "Batch");
+                 check(bolt_batch_create(b->ctx, (int)n, cc.data(), coll, of.data(),
+                                         tracer ? omega_g.data() : nullptr, tracer ? alpha.data() : nullptr,
+                                         tracer ? c0.data() : nullptr, tracer ? gg.data() : nullptr,
+                                         &b->batch), "Batch");
                  b->shapes = shapes;
                  return PyBatch{b};
              }),
              py::arg("dev"), py::arg("counts"), py::arg("collision"), py::arg("omega_f"),
-             py::arg("omega_g"), py::arg("alpha"), py::arg("c0"), py::arg("g"))
+             py::arg("omega_g") = py::none(), py::arg("alpha") = py::none(),
+             py::arg("c0") = py::none(), py::arg("g") = py::none())
+        .def("residual", [](PyBatch& b, int k, uint64_t steps_between) {
+            double r = 0;
+            check(bolt_batch_residual(b.core->batch, k, steps_between, &r), "residual");
+            return r;
+        }, py::arg("inst"), py::arg("steps_between"))
         .def_property_readonly("n", [](PyBatch& b) { return b.core->shapes.size(); })
         .def_property_readonly("cells", [](PyBatch& b) {
             uint64_t c = 0;
@@ -642,6 +681,91 @@ PYBIND11_MODULE(_boltzmann, m) {
             check(bolt_batch_reduce(b.core->batch, k, w, out), "reduce");
             return out[0];
         }, py::arg("inst"), py::arg("what"));
+
+    // ---- the warm-start network ----
+    py::class_<PyNet>(m, "Network")
+        .def(py::init([](const std::string& dev, int nx, int ny, int batch, int width, uint64_t seed) {
+                 auto n = std::make_shared<NetCore>();
+                 check(bolt_ctx_create(device_kind(dev), 0, &n->ctx), "device");
+                 check(bolt_nn_create(n->ctx, nx, ny, batch, width, seed, &n->net), "Network");
+                 n->nx = nx;
+                 n->ny = ny;
+                 n->batch = batch;
+                 return PyNet{n};
+             }),
+             py::arg("dev"), py::arg("nx"), py::arg("ny"), py::arg("batch"), py::arg("width") = 32,
+             py::arg("seed") = 0)
+        .def_property_readonly("param_count", [](PyNet& n) {
+            uint64_t c = 0;
+            check(bolt_nn_param_count(n.core->net, &c), "param_count");
+            return c;
+        })
+        .def("get_params", [](PyNet& n) {
+            uint64_t c = 0;
+            check(bolt_nn_param_count(n.core->net, &c), "param_count");
+            py::array_t<float> out((ssize_t)c);
+            check(bolt_nn_get_params(n.core->net, out.mutable_data(), c), "get_params");
+            return out;
+        })
+        .def("set_params", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+            check(bolt_nn_set_params(n.core->net, a.data(), (uint64_t)a.size()), "set_params");
+        })
+        .def("loss_grad", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> x,
+                             py::array_t<float, py::array::c_style | py::array::forcecast> y) {
+            NetCore& c = *n.core;
+            const size_t pix = (size_t)c.batch * c.nx * c.ny;
+            if ((size_t)x.size() != pix * 4 || (size_t)y.size() != pix * 3)
+                throw BoltError("loss_grad: x must be [batch, nx, ny, 4] and y [batch, nx, ny, 3]");
+            uint64_t np_ = 0;
+            check(bolt_nn_param_count(c.net, &np_), "param_count");
+            py::array_t<float> g((ssize_t)np_);
+            double loss = 0;
+            check(bolt_nn_loss_grad(c.net, x.data(), y.data(), &loss, g.mutable_data(), np_), "loss_grad");
+            return py::make_tuple(loss, g);
+        })
+        .def("set_data", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> x,
+                            py::array_t<float, py::array::c_style | py::array::forcecast> y) {
+            NetCore& c = *n.core;
+            const size_t pix = (size_t)c.nx * c.ny;
+            const size_t cnt = (size_t)x.size() / (pix * 4);
+            if (cnt * pix * 4 != (size_t)x.size() || (size_t)y.size() != cnt * pix * 3)
+                throw BoltError("set_data: x must be [n, nx, ny, 4] and y [n, nx, ny, 3]");
+            check(bolt_nn_set_data(c.net, (int)cnt, x.data(), y.data()), "set_data");
+        })
+        .def("train", [](PyNet& n, int epochs, float lr, uint64_t seed) {
+            py::array_t<float> losses(epochs);
+            check(bolt_nn_train(n.core->net, epochs, lr, seed, losses.mutable_data()), "train");
+            return losses;
+        }, py::arg("epochs"), py::arg("lr") = 1e-3f, py::arg("seed") = 0)
+        .def("predict", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> x) {
+            NetCore& c = *n.core;
+            const size_t pix = (size_t)c.nx * c.ny;
+            const size_t cnt = (size_t)x.size() / (pix * 4);
+            if (cnt * pix * 4 != (size_t)x.size() || cnt == 0) throw BoltError("predict: x must be [n, nx, ny, 4]");
+            py::array_t<float> y(std::vector<ssize_t>{(ssize_t)cnt, c.nx, c.ny, 3});
+            check(bolt_nn_predict(c.net, (int)cnt, x.data(), y.mutable_data()), "predict");
+            return y;
+        });
+
+    m.def("gemm_check", [](const std::string& dev, int a_form, int b_form,
+                           py::array_t<float, py::array::c_style | py::array::forcecast> a,
+                           py::array_t<float, py::array::c_style | py::array::forcecast> b, int k_split) {
+        // C[M][N] = A.B^T through the network GEMM; a is [M][K] (a_form 0) or [K][M] (1),
+        // b is [N][K] (b_form 0) or [K][N] (1)
+        if (a.ndim() != 2 || b.ndim() != 2) throw BoltError("gemm_check: 2D arrays");
+        const int M = (int)(a_form == 0 ? a.shape(0) : a.shape(1));
+        const int K = (int)(a_form == 0 ? a.shape(1) : a.shape(0));
+        const int N = (int)(b_form == 0 ? b.shape(0) : b.shape(1));
+        const int Kb = (int)(b_form == 0 ? b.shape(1) : b.shape(0));
+        if (K != Kb) throw BoltError("gemm_check: K mismatch");
+        bolt_ctx* ctx = nullptr;
+        check(bolt_ctx_create(device_kind(dev), 0, &ctx), "device");
+        py::array_t<float> c(std::vector<ssize_t>{M, N});
+        const int rc = bolt_nn_gemm_check(ctx, a_form, b_form, M, N, K, a.data(), b.data(), c.mutable_data(), k_split);
+        bolt_ctx_destroy(ctx);
+        check(rc, "gemm_check");
+        return c;
+    }, py::arg("dev"), py::arg("a_form"), py::arg("b_form"), py::arg("a"), py::arg("b"), py::arg("k_split") = 0);
 
     m.def("version", []() { return std::string(bolt_version()); });
 }

@@ -11,6 +11,7 @@
 //   vel          SoA, vel + a * cells
 #include <hip/hip_runtime.h>
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -25,12 +26,14 @@
 
 using namespace bolt;
 
+#include "runtime_internal.hpp"
+
 // ---------------------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const char* fmt, ...) {
+int bolt_fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -40,22 +43,7 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                    \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(BOLT_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                             \
-    } while (0)
-
-// ---------------------------------------------------------------------------------------
-// objects
-// ---------------------------------------------------------------------------------------
-struct bolt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string info;
-};
+const char* bolt_error_text() { return g_last_error.c_str(); }
 
 struct Tracer {
     int q = 0;
@@ -93,6 +81,7 @@ struct bolt_lattice {
     double* result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t device_bytes = 0;
+    double* vel_snap = nullptr;  // velocity snapshot for the steady-state residual
 };
 
 static size_t store_size(int s) {
@@ -507,6 +496,7 @@ int bolt_lattice_destroy(bolt_lattice* L) {
     }
     hipFree(L->partials);
     hipFree(L->result);
+    hipFree(L->vel_snap);
     if (L->ev0) hipEventDestroy(L->ev0);
     if (L->ev1) hipEventDestroy(L->ev1);
     delete L;
@@ -949,6 +939,87 @@ int bolt_lattice_set_block(bolt_lattice* L, int block) {
     return BOLT_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Steady-state residual, double precision:
+//   r = max_x |u - u_snap| / (k * max_x |u|)   over non-wall cells, |.| the Euclidean norm,
+// where u_snap is the velocity saved by the previous call and k the steps between them.
+// ---------------------------------------------------------------------------------------
+__device__ inline double block_max(double v) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+template <typename R>
+__global__ void k_residual(const R* vel, double* snap, const uint8_t* flags, long long ncell,
+                           int d, int have_snap, double* partials) {
+    double dmax = 0.0, umax = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        double du2 = 0.0, u2 = 0.0;
+        for (int a = 0; a < d; ++a) {
+            const double u = (double)vel[(long long)a * ncell + c];
+            const double w = u - snap[(long long)a * ncell + c];
+            du2 += w * w;
+            u2 += u * u;
+            snap[(long long)a * ncell + c] = u;
+        }
+        if (!(flags[c] & 1)) {
+            dmax = fmax(dmax, du2);
+            umax = fmax(umax, u2);
+        }
+    }
+    const double a = block_max(have_snap ? dmax : 0.0);
+    const double b = block_max(umax);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ void k_max_pairs(const double* partials, int n, double* out) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        a = fmax(a, partials[2 * k]);
+        b = fmax(b, partials[2 * k + 1]);
+    }
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(a);
+        out[1] = sqrt(b);
+    }
+}
+
+int bolt_lattice_residual(bolt_lattice* L, uint64_t steps_between, double* out) {
+    if (!L || !out) return fail(BOLT_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    hipStream_t s = L->ctx->stream;
+    int have = L->vel_snap != nullptr;
+    if (!have) {
+        int rc = dev_alloc(L, (void**)&L->vel_snap, (size_t)L->ncell * L->ndim * sizeof(double));
+        if (rc) return rc;
+    }
+    constexpr int kBlocks = kRedBlocks / 2;
+    if (is_f64(L))
+        hipLaunchKernelGGL(k_residual<double>, dim3(kBlocks), dim3(256), 0, s, (const double*)L->vel, L->vel_snap, L->flags, L->ncell, L->ndim, have, L->partials);
+    else
+        hipLaunchKernelGGL(k_residual<float>, dim3(kBlocks), dim3(256), 0, s, (const float*)L->vel, L->vel_snap, L->flags, L->ncell, L->ndim, have, L->partials);
+    hipLaunchKernelGGL(k_max_pairs, dim3(1), dim3(256), 0, s, L->partials, kBlocks, L->result);
+    HIP_TRY(hipGetLastError());
+    double r[2];
+    HIP_TRY(hipMemcpyAsync(r, L->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!have || steps_between == 0 || r[1] == 0.0) *out = HUGE_VAL;
+    else *out = r[0] / ((double)steps_between * r[1]);
+    return BOLT_OK;
+}
+
 // =======================================================================================
 // Batches: many independent small D2Q9 + D2Q5 Boussinesq lattices advanced together.
 //
@@ -963,7 +1034,7 @@ int bolt_lattice_set_block(bolt_lattice* L, int block) {
 // =======================================================================================
 struct InstDesc {
     char* f;            // plane 0 of the fluid populations
-    char* g;            // plane 0 of the tracer populations
+    char* g;            // plane 0 of the tracer populations (null: fluid only)
     void* rho;
     void* vel0;
     void* vel1;
@@ -987,7 +1058,7 @@ struct BatchParams {
     int write_macro;
 };
 
-template <int COLL, bool EVEN>
+template <int COLL, bool EVEN, int TQ>
 __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
     const Tile t = bp.tiles[b];
@@ -1001,8 +1072,10 @@ __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     p.len = d.ny;
 #pragma unroll
     for (int i = 0; i < 9; ++i) p.fpl[i] = d.f + (size_t)i * d.plane_bytes;
+    if constexpr (TQ != 0) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) p.gpl[i] = d.g + (size_t)i * d.plane_bytes;
+        for (int i = 0; i < TQ; ++i) p.gpl[i] = d.g + (size_t)i * d.plane_bytes;
+    }
     p.rho = d.rho;
     p.vel[0] = d.vel0;
     p.vel[1] = d.vel1;
@@ -1018,7 +1091,7 @@ __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     p.write_macro = bp.write_macro;
     p.tC = d.C;
     p.t_om = d.om_g;
-    update_cell<2, 9, COLL, kF32, EVEN, 5>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
+    update_cell<2, 9, COLL, kF32, EVEN, TQ>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
 }
 
 struct bolt_batch {
@@ -1034,6 +1107,7 @@ struct bolt_batch {
     int graph_block = 0, graph_len = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t cells = 0;
+    bool tracers = true;
 };
 
 static void batch_drop_graph(bolt_batch* B) {
@@ -1063,13 +1137,15 @@ static int batch_build_tiles(bolt_batch* B, int block) {
 static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
     BatchParams bp{B->d_inst, B->d_tiles, B->ntiles, write_macro ? 1 : 0};
     const dim3 grid((unsigned)B->ntiles), blk((unsigned)B->block);
-    if (B->collision == BOLT_BGK) {
-        if (even) hipLaunchKernelGGL((rb_batch_step<0, true>), grid, blk, 0, s, bp);
-        else hipLaunchKernelGGL((rb_batch_step<0, false>), grid, blk, 0, s, bp);
+#define BOLT_BATCH_LAUNCH(C, E, TQ) hipLaunchKernelGGL((rb_batch_step<C, E, TQ>), grid, blk, 0, s, bp)
+    if (B->tracers) {
+        if (B->collision == BOLT_BGK) { if (even) BOLT_BATCH_LAUNCH(0, true, 5); else BOLT_BATCH_LAUNCH(0, false, 5); }
+        else { if (even) BOLT_BATCH_LAUNCH(1, true, 5); else BOLT_BATCH_LAUNCH(1, false, 5); }
     } else {
-        if (even) hipLaunchKernelGGL((rb_batch_step<1, true>), grid, blk, 0, s, bp);
-        else hipLaunchKernelGGL((rb_batch_step<1, false>), grid, blk, 0, s, bp);
+        if (B->collision == BOLT_BGK) { if (even) BOLT_BATCH_LAUNCH(0, true, 0); else BOLT_BATCH_LAUNCH(0, false, 0); }
+        else { if (even) BOLT_BATCH_LAUNCH(1, true, 0); else BOLT_BATCH_LAUNCH(1, false, 0); }
     }
+#undef BOLT_BATCH_LAUNCH
 }
 
 static int batch_capture(bolt_batch* B, int len) {
@@ -1113,18 +1189,18 @@ static int batch_upload_desc(bolt_batch* B) {
         bolt_lattice* L = B->lat[k];
         InstDesc& d = h[k];
         d.f = (char*)L->f;
-        d.g = (char*)L->tracers[0].g;
+        d.g = L->tracers.empty() ? nullptr : (char*)L->tracers[0].g;
         d.rho = L->rho;
         d.vel0 = L->vel;
         d.vel1 = (char*)L->vel + (size_t)L->ncell * 4;
-        d.C = L->tracers[0].C;
+        d.C = L->tracers.empty() ? nullptr : L->tracers[0].C;
         d.flags = L->flags;
         d.plane_bytes = (uint32_t)(L->stride * 4);
         d.nx = L->n[0];
         d.ny = L->n[1];
         d.om_p = L->om_p;
         d.om_n = L->om_n;
-        d.om_g = L->tracers[0].omega;
+        d.om_g = L->tracers.empty() ? 0.f : L->tracers[0].omega;
         if (!L->couplings.empty()) {
             const Coupling& c = L->couplings.back();
             d.acc_mode = kAccBoussinesq;
@@ -1148,13 +1224,14 @@ static int batch_upload_desc(bolt_batch* B) {
 int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision,
                       const float* omega_f, const float* omega_g, const float* alpha,
                       const float* c0, const float* g, bolt_batch** out) {
-    if (!ctx || !counts || !omega_f || !omega_g || !alpha || !c0 || !g || !out)
-        return fail(BOLT_EINVAL, "null argument");
+    if (!ctx || !counts || !omega_f || !out) return fail(BOLT_EINVAL, "null argument");
+    if (omega_g && (!alpha || !c0 || !g)) return fail(BOLT_EINVAL, "null coupling argument");
     *out = nullptr;
     if (n < 1 || n > 65535) return fail(BOLT_EINVAL, "instance count %d out of range", n);
     auto B = std::make_unique<bolt_batch>();
     B->ctx = ctx;
     B->collision = collision;
+    B->tracers = omega_g != nullptr;
     for (int k = 0; k < n; ++k) {
         if (counts[2 * k] > 65535) {
             for (auto* L : B->lat) bolt_lattice_destroy(L);
@@ -1165,10 +1242,10 @@ int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision
                                      omega_f[2 * k + 1], BOLT_STORE_F32, &L);
         int tid = 0;
         if (!rc) B->lat.push_back(L);
-        if (!rc) rc = bolt_lattice_add_tracer(L, 5, omega_g[k], &tid);
-        if (!rc) rc = bolt_lattice_add_coupling(L, tid, alpha[k], c0[k], g + 2 * k);
+        if (!rc && omega_g) rc = bolt_lattice_add_tracer(L, 5, omega_g[k], &tid);
+        if (!rc && omega_g) rc = bolt_lattice_add_coupling(L, tid, alpha[k], c0[k], g + 2 * k);
         if (rc) {
-            std::string msg = std::string("instance ") + std::to_string(k) + ": " + g_last_error;
+            std::string msg = std::string("instance ") + std::to_string(k) + ": " + bolt_error_text();
             for (auto* l : B->lat) bolt_lattice_destroy(l);
             return fail(rc, "%s", msg.c_str());
         }
@@ -1292,8 +1369,8 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     int rc = BOLT_OK;
     for (auto* L : B->lat) {
         if (!rc) rc = keep(L->f, (size_t)L->stride * 9 * 4);
-        if (!rc) rc = keep(L->tracers[0].g, (size_t)L->stride * 5 * 4);
-        if (!rc) rc = keep(L->tracers[0].C, (size_t)L->ncell * 4);
+        if (!rc && B->tracers) rc = keep(L->tracers[0].g, (size_t)L->stride * 5 * 4);
+        if (!rc && B->tracers) rc = keep(L->tracers[0].C, (size_t)L->ncell * 4);
         if (!rc) rc = keep(L->rho, (size_t)L->ncell * 4);
         if (!rc) rc = keep(L->vel, (size_t)L->ncell * 2 * 4);
     }
@@ -1335,4 +1412,10 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     rep += line;
     if (report && len) snprintf(report, len, "%s", rep.c_str());
     return rc;
+}
+
+int bolt_batch_residual(bolt_batch* B, int inst, uint64_t steps_between, double* out) {
+    bolt_lattice* L;
+    int rc = batch_lat(B, inst, &L);
+    return rc ? rc : bolt_lattice_residual(L, steps_between, out);
 }

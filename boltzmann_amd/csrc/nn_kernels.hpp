@@ -1,0 +1,315 @@
+// Kernels of the warm-start network: a bf16 MFMA GEMM with LDS staging, im2col / col2im for
+// 3x3 convolutions, and the element-wise layers.  Activations are channels-last bf16
+// ([batch][nx][ny][C], C contiguous); gradients and master weights are float.
+//
+// The GEMM computes C(m, n) = sum_k A(m, k) B(n, k) with
+//   A in "MK" form (A(m,k) at a[m*lda + k]) or "KM" form (a[k*lda + m]),
+//   B in "NK" form (B(n,k) at b[n*ldb + k]) or "KN" form (b[k*ldb + n]),
+// which covers the forward convolution (im2col(X) x W^T), the input gradient
+// (dZ x W -> col2im) and the weight gradient (dZ^T x im2col(X)).  Work-group tile 64 x 64,
+// K step 32, four waves each owning a 32 x 32 block computed with
+// v_mfma_f32_32x32x16_bf16, float accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nnk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t u16;
+
+__device__ inline u16 bf16_bits(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ inline float bf16_float(u16 b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+enum : int { kMK = 0, kKM = 1 };             // A forms
+enum : int { kNK = 0, kKN = 1 };             // B forms
+enum : int { kStoreF32 = 0, kStoreBF16 = 1, kAtomicF32 = 2 };
+
+struct GemmArgs {
+    const u16* a;
+    const u16* b;
+    void* c;
+    const float* bias;   // per column n, or null
+    int M, N, K;
+    int lda, ldb, ldc;
+    int c_off;           // column offset into c (channel slice of a concatenation)
+    int relu;            // apply max(0, .) in the epilogue
+    int k_split;         // K range per z-slice (multiple of 32)
+};
+
+constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8, kLdsRow = kTK + kPad;
+
+// Loads a 64 x 32 tile of an operand into LDS as [row][k] (k contiguous), zero-filled
+// outside the matrix.  kform: 0 -> element (row, k) at p[row*ld + k]; 1 -> at p[k*ld + row].
+template <int FORM>
+__device__ inline void load_tile(u16* lds, const u16* __restrict__ p, int ld, int row0, int rows,
+                                 int k0, int kend) {
+    const int t = threadIdx.x;  // 256 threads, 8 elements each
+    if (FORM == 0) {
+        const int r = t >> 2, kc = (t & 3) * 8;
+        const int gr = row0 + r, gk = k0 + kc;
+        u16 v[8];
+        if (gr < rows && gk + 8 <= kend && ((ld & 7) == 0)) {
+            const uint4 q = *(const uint4*)(p + (size_t)gr * ld + gk);
+            *(uint4*)v = q;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = (gr < rows && gk + e < kend) ? p[(size_t)gr * ld + gk + e] : (u16)0;
+        }
+        *(uint4*)(lds + r * kLdsRow + kc) = *(uint4*)v;
+    } else {
+        const int k = t >> 3, rc = (t & 7) * 8;
+        const int gk = k0 + k, gr = row0 + rc;
+        u16 v[8];
+        if (gk < kend && gr + 8 <= rows && ((ld & 7) == 0)) {
+            const uint4 q = *(const uint4*)(p + (size_t)gk * ld + gr);
+            *(uint4*)v = q;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[e] = (gk < kend && gr + e < rows) ? p[(size_t)gk * ld + gr + e] : (u16)0;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) lds[(rc + e) * kLdsRow + k] = v[e];
+    }
+}
+
+template <int AF, int BF, int OUT>
+__global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) u16 sa[kTM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) u16 sb[kTN * kLdsRow];
+    const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+    const int kb = blockIdx.z * g.k_split;
+    const int ke = min(g.K, kb + g.k_split);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x16 acc = {0};
+    for (int k0 = kb; k0 < ke; k0 += kTK) {
+        load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke);
+        load_tile<BF>(sb, g.b, g.ldb, n0, g.N, k0, ke);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kTK; kk += 16) {
+            // A(i, k): lane holds row i = lane % 32, k = 8 * (lane / 32) .. + 7
+            const bf16x8 a = *(const bf16x8*)(sa + (wm + (lane & 31)) * kLdsRow + kk + 8 * (lane >> 5));
+            const bf16x8 b = *(const bf16x8*)(sb + (wn + (lane & 31)) * kLdsRow + kk + 8 * (lane >> 5));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D(i, j): lane holds column j = lane % 32, rows (r/4)*8 + (lane/32)*4 + r%4
+    const int j = n0 + wn + (lane & 31);
+    if (j >= g.N) return;
+    const float bj = g.bias ? g.bias[j] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = m0 + wm + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (i >= g.M) continue;
+        float v = acc[r] + bj;
+        if (g.relu) v = fmaxf(v, 0.f);
+        const size_t o = (size_t)i * g.ldc + g.c_off + j;
+        if (OUT == kStoreF32) ((float*)g.c)[o] = v;
+        else if (OUT == kStoreBF16) ((u16*)g.c)[o] = bf16_bits(v);
+        else atomicAdd((float*)g.c + o, v);
+    }
+}
+
+// im2col for a 3x3, padding-1 convolution: col[p][tap * C + c] = X[b, x+dx, y+dy, c] (0 off
+// the image), tap = (dx+1)*3 + (dy+1), p = (b*nx + x)*ny + y.  X has channel stride ldx.
+__global__ void im2col3x3(const u16* __restrict__ X, int ldx, int xoff, u16* __restrict__ col,
+                          int nb, int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nb * nx * ny * 9 * (C / 8);
+    if (t >= total) return;
+    const int c8 = (int)(t % (C / 8));
+    long long r = t / (C / 8);
+    const int tap = (int)(r % 9);
+    const long long p = r / 9;
+    const int y = (int)(p % ny);
+    const long long bx = p / ny;
+    const int x = (int)(bx % nx);
+    const int b = (int)(bx / nx);
+    const int xx = x + tap / 3 - 1, yy = y + tap % 3 - 1;
+    uint4 v = {0, 0, 0, 0};
+    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny)
+        v = *(const uint4*)(X + ((size_t)((size_t)b * nx + xx) * ny + yy) * ldx + xoff + c8 * 8);
+    *(uint4*)(col + (size_t)p * 9 * C + tap * C + c8 * 8) = v;
+}
+
+// col2im (gather): dX[p][c] (+)= sum_tap dcol[p - shift(tap)][tap * C + c]
+__global__ void col2im3x3(const float* __restrict__ dcol, float* __restrict__ dX, int lddx,
+                          int dxoff, int accumulate, int nb, int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nb * nx * ny * C;
+    if (t >= total) return;
+    const int c = (int)(t % C);
+    const long long p = t / C;
+    const int y = (int)(p % ny);
+    const long long bx = p / ny;
+    const int x = (int)(bx % nx);
+    const int b = (int)(bx / nx);
+    float s = 0.f;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        // the output pixel (x - dx, y - dy) read this pixel through tap (dx, dy)
+        const int ox = x - (tap / 3 - 1), oy = y - (tap % 3 - 1);
+        if (ox >= 0 && ox < nx && oy >= 0 && oy < ny)
+            s += dcol[((size_t)((size_t)b * nx + ox) * ny + oy) * 9 * C + tap * C + c];
+    }
+    float* o = dX + (size_t)p * lddx + dxoff + c;
+    *o = accumulate ? *o + s : s;
+}
+
+// dZ = dY * (Y > 0) as bf16, dY float with channel stride ld / offset off
+__global__ void relu_grad(const float* __restrict__ dY, int ldy, int yoff, const u16* __restrict__ Y,
+                          int ldY, int Yoff, u16* __restrict__ dZ, long long P, int C, int relu) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * C) return;
+    const long long p = t / C;
+    const int c = (int)(t % C);
+    float g = dY[p * ldy + yoff + c];
+    if (relu && !(bf16_float(Y[p * ldY + Yoff + c]) > 0.f)) g = 0.f;
+    dZ[t] = bf16_bits(g);
+}
+
+// dbias[c] += sum_p dZ[p][c]
+__global__ void bias_grad(const u16* __restrict__ dZ, long long P, int C, float* __restrict__ db) {
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (long long p = threadIdx.x; p < P; p += blockDim.x) s += bf16_float(dZ[p * C + c]);
+    __shared__ float sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] += sh[0];
+}
+
+// 2x2 average pooling (channels-last), ld/off select a channel slice of X
+__global__ void avgpool2(const u16* __restrict__ X, int ldx, int xoff, u16* __restrict__ Y, int nb,
+                         int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ox = nx / 2, oy = ny / 2;
+    if (t >= (long long)nb * ox * oy * C) return;
+    const int c = (int)(t % C);
+    long long r = t / C;
+    const int y = (int)(r % oy);
+    r /= oy;
+    const int x = (int)(r % ox);
+    const int b = (int)(r / ox);
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            s += bf16_float(X[((size_t)((size_t)b * nx + 2 * x + i) * ny + 2 * y + j) * ldx + xoff + c]);
+    Y[t] = bf16_bits(0.25f * s);
+}
+
+// backward of avgpool2: dX[b, x, y, c] (+)= dY[b, x/2, y/2, c] / 4
+__global__ void avgpool2_grad(const float* __restrict__ dY, float* __restrict__ dX, int lddx,
+                              int dxoff, int nb, int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * nx * ny * C) return;
+    const int c = (int)(t % C);
+    const long long p = t / C;
+    const int y = (int)(p % ny);
+    const long long bx = p / ny;
+    const int x = (int)(bx % nx);
+    const int b = (int)(bx / nx);
+    const float g = 0.25f * dY[((size_t)((size_t)b * (nx / 2) + x / 2) * (ny / 2) + y / 2) * C + c];
+    dX[(size_t)p * lddx + dxoff + c] += g;
+}
+
+// nearest 2x upsampling into a channel slice: Y[b, x, y, off + c] = X[b, x/2, y/2, c]
+__global__ void upsample2(const u16* __restrict__ X, u16* __restrict__ Y, int ldy, int yoff, int nb,
+                          int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * nx * ny * C) return;
+    const int c = (int)(t % C);
+    const long long p = t / C;
+    const int y = (int)(p % ny);
+    const long long bx = p / ny;
+    const int x = (int)(bx % nx);
+    const int b = (int)(bx / nx);
+    Y[(size_t)p * ldy + yoff + c] = X[((size_t)((size_t)b * (nx / 2) + x / 2) * (ny / 2) + y / 2) * C + c];
+}
+
+// backward of upsample2: dX[b, x, y, c] = sum of the four dY[.., off + c]
+__global__ void upsample2_grad(const float* __restrict__ dY, int ldy, int yoff, float* __restrict__ dX,
+                               int nb, int nx, int ny, int C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ox = nx / 2, oy = ny / 2;
+    if (t >= (long long)nb * ox * oy * C) return;
+    const int c = (int)(t % C);
+    long long r = t / C;
+    const int y = (int)(r % oy);
+    r /= oy;
+    const int x = (int)(r % ox);
+    const int b = (int)(r / ox);
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            s += dY[((size_t)((size_t)b * nx + 2 * x + i) * ny + 2 * y + j) * ldy + yoff + c];
+    dX[t] = s;
+}
+
+// mean-squared error over all elements; dP = 2 (P - T) / n; partial sums per block
+__global__ void mse_loss(const float* __restrict__ P, const float* __restrict__ T, float* __restrict__ dP,
+                         long long n, double* __restrict__ partial) {
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float d = P[i] - T[i];
+        s += (double)d * d;
+        dP[i] = 2.0f * d / (float)n;
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Adam with bias correction; refreshes the bf16 copy of the weights
+__global__ void adam_step(float* __restrict__ w, u16* __restrict__ wb, const float* __restrict__ g,
+                          float* __restrict__ m, float* __restrict__ v, long long n, float lr,
+                          float b1, float b2, float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = bf16_bits(wi);
+}
+
+__global__ void to_bf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = bf16_bits(x[i]);
+}
+
+__global__ void fill_f32(float* __restrict__ x, long long n, float v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+}  // namespace nnk

@@ -1,0 +1,259 @@
+"""Steady channel flows past obstacles, solved in batches, and the learned warm start.
+
+Case (lattice units): D2Q9 TRT (magic number 1/4) channel of nx x ny cells (x along the
+channel, y across, y contiguous); walls on y = 0 and y = ny-1; the first column holds the
+inflow (fixed fluid: rho = 1, u = (u_in, 0)); the last column the outflow (fixed density
+rho = 1; like every fixed cell of the reference's kernels it also keeps its initial
+velocity, lib.cl:257-262); 1 to 8 circles or rectangles of radius / half-size 8..32 as
+walls.  Re = u_in D / nu on D = twice the largest obstacle size sets tau = 3 nu + 1/2.
+
+The plain solver is the judge: a run has converged when the FP64 residual
+max|u^n - u^(n-100)| / (100 max|u^n|) drops below the tolerance (1e-6).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Batch
+from .core import CellFlags
+
+
+@dataclass
+class Case:
+    seed: int
+    nx: int
+    ny: int
+    u_in: float
+    re: float
+    tau: float
+    mask: np.ndarray = field(repr=False)   # bool [nx, ny]: obstacle or wall
+
+    @property
+    def omega(self):
+        wp = 1.0 / self.tau
+        wn = 1.0 / (0.25 / (self.tau - 0.5) + 0.5)
+        return (wp, wn)
+
+    def flags(self) -> np.ndarray:
+        f = np.zeros((self.nx, self.ny), np.int32)
+        f[0, :] = CellFlags.FIXED_FLUID
+        f[-1, :] = CellFlags.FIXED_FLUID_DENSITY
+        f[self.mask] = CellFlags.WALL
+        return f
+
+    def cold_state(self):
+        """rho = 1, u = (u_in, 0) outside walls (the usual plug-flow start)."""
+        rho = np.ones((self.nx, self.ny), np.float32)
+        vel = np.zeros((self.nx, self.ny, 2), np.float32)
+        vel[..., 0] = self.u_in
+        vel[self.mask] = 0.0
+        return rho, vel
+
+    def features(self) -> np.ndarray:
+        """Network input [4, nx, ny]: obstacle mask, inflow speed / 0.05, tau - 1/2 scaled,
+        distance along the channel."""
+        x = np.linspace(0.0, 1.0, self.nx, dtype=np.float32)[:, None]
+        return np.stack([self.mask.astype(np.float32),
+                         np.full((self.nx, self.ny), self.u_in / 0.05, np.float32),
+                         np.full((self.nx, self.ny), (self.tau - 0.5) * 2.0, np.float32),
+                         np.broadcast_to(x, (self.nx, self.ny)).astype(np.float32)])
+
+
+def make_case(seed: int, nx: int = 512, ny: int = 256, re_range=(10.0, 100.0),
+              size_range=(8, 32)) -> Case:
+    rng = np.random.default_rng(seed)
+    u_in = float(rng.uniform(0.02, 0.05))
+    nobs = int(rng.integers(1, 9))
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    mask = np.zeros((nx, ny), bool)
+    mask[:, 0] = mask[:, -1] = True
+    lo, hi = size_range
+    lo = max(2, min(lo, ny // 6))
+    hi = max(lo + 1, min(hi, ny // 4))
+    biggest = 0
+    for _ in range(nobs):
+        r = int(rng.integers(lo, hi + 1))
+        biggest = max(biggest, r)
+        cx = int(rng.integers(nx // 8 + r, int(nx * 0.6) - r))
+        cy = int(rng.integers(r + 4, ny - r - 4))
+        if rng.random() < 0.5:
+            mask |= (X - cx) ** 2 + (Y - cy) ** 2 < r * r
+        else:
+            b = int(rng.integers(lo, hi + 1))
+            mask |= (np.abs(X - cx) < r) & (np.abs(Y - cy) < b)
+    re = float(rng.uniform(*re_range))
+    nu = u_in * 2 * biggest / re
+    return Case(seed, nx, ny, u_in, re, 3 * nu + 0.5, mask)
+
+
+def _batch(cases, dev="gpu") -> Batch:
+    return Batch(dev, [(c.nx, c.ny) for c in cases], "trt", [c.omega for c in cases])
+
+
+def load_state(b: Batch, k: int, case: Case, rho: np.ndarray, vel: np.ndarray) -> None:
+    """Writes flags and an initial (rho, u); fixed cells keep their prescribed values."""
+    cold_rho, cold_vel = case.cold_state()
+    flags = case.flags()
+    fixed = (flags & 6) != 0
+    rho = np.where(fixed, cold_rho, rho).astype(np.float32)
+    vel = np.where(fixed[..., None], cold_vel, vel).astype(np.float32)
+    vel[case.mask] = 0.0
+    b.write(k, "flags", flags)
+    b.write(k, "rho", rho)
+    b.write(k, "vel", vel)
+
+
+@dataclass
+class SolveResult:
+    steps: np.ndarray        # steps to reach tol (-1: not reached)
+    residual: np.ndarray     # last residual per case
+    seconds: float
+    first_residual: np.ndarray
+
+
+def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
+          batch: Batch | None = None, stop_when_done=True, keep_fields=True):
+    """Runs every case to the residual tolerance in one batch.
+
+    init: None for the cold start, else a list of (rho, vel) per case.  Returns the
+    SolveResult and, with keep_fields, the final (rho, vel) of every case."""
+    b = batch or _batch(cases, dev)
+    for k, c in enumerate(cases):
+        if init is None:
+            load_state(b, k, c, *c.cold_state())
+        else:
+            load_state(b, k, c, *init[k])
+    b.equilibrate()
+    n = len(cases)
+    steps = np.full(n, -1, np.int64)
+    last = np.full(n, np.inf)
+    first = np.full(n, np.nan)
+    t0 = time.perf_counter()
+    done = 0
+    while done < max_steps:
+        b.step(check_every - 100)
+        for k in range(n):
+            b.residual(k, 0)          # snapshot u^(n-100)
+        b.step(100)
+        done += check_every
+        for k in range(n):
+            r = b.residual(k, 100)
+            last[k] = r
+            if np.isnan(first[k]):
+                first[k] = r
+            if steps[k] < 0 and r <= tol:
+                steps[k] = done
+        if stop_when_done and (steps >= 0).all():
+            break
+    secs = time.perf_counter() - t0
+    res = SolveResult(steps, last, secs, first)
+    if not keep_fields:
+        return res, None
+    fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
+    return res, fields
+
+
+# ---- learned warm start ----------------------------------------------------------------
+def to_targets(case: Case, rho: np.ndarray, vel: np.ndarray) -> np.ndarray:
+    """Network targets [nx, ny, 3]: u / u_in (two components) and (rho - 1) / u_in^2."""
+    t = np.empty((case.nx, case.ny, 3), np.float32)
+    t[..., 0] = vel[..., 0] / case.u_in
+    t[..., 1] = vel[..., 1] / case.u_in
+    t[..., 2] = (rho - 1.0) / case.u_in ** 2
+    return t
+
+
+def from_targets(case: Case, t: np.ndarray):
+    rho = (1.0 + t[..., 2] * case.u_in ** 2).astype(np.float32)
+    vel = (t[..., :2] * case.u_in).astype(np.float32)
+    vel[case.mask] = 0.0
+    return rho, vel
+
+
+def features(cases) -> np.ndarray:
+    """[n, nx, ny, 4] channels-last network inputs."""
+    return np.stack([c.features().transpose(1, 2, 0) for c in cases]).astype(np.float32)
+
+
+def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
+             check_every: int = 2000, tol: float = 1e-6, log=None, **case_kw):
+    """Solves n cases (seeds seed0 ..) to the tolerance in batches; returns the converged
+    cases, their steady (rho, vel) and the steps they took."""
+    cases, fields, steps = [], [], []
+    for b0 in range(0, n, batch):
+        cs = [make_case(seed0 + s, **case_kw) for s in range(b0, min(n, b0 + batch))]
+        res, fl = solve(cs, tol=tol, check_every=check_every, max_steps=max_steps)
+        for c, f, st in zip(cs, fl, res.steps):
+            if st >= 0 and np.isfinite(f[1]).all():
+                cases.append(c)
+                fields.append(f)
+                steps.append(int(st))
+        if log:
+            log(f"generated {b0 + len(cs)}/{n} cases, {len(cases)} converged, "
+                f"{res.seconds:.1f} s for this batch")
+    return cases, fields, np.array(steps)
+
+
+def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
+          seed: int = 0, dev: str = "gpu"):
+    """Trains the network on solver-generated steady states with the library's kernels."""
+    from . import Network
+    c0 = cases[0]
+    net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed)
+    x = features(cases)
+    y = np.stack([to_targets(c, *f) for c, f in zip(cases, fields)])
+    net.set_data(x, y)
+    losses = net.train(epochs, lr=lr, seed=seed)
+    return net, losses
+
+
+@dataclass
+class Comparison:
+    cold_steps: np.ndarray
+    warm_steps: np.ndarray       # steps of the warm run, or of warm + cold after a fallback
+    fallback: np.ndarray         # warm start rejected (its first residual was worse)
+    cold_seconds: float
+    warm_seconds: float          # inference + solve (+ the fallback's cold re-run if any)
+    inference_seconds: float
+
+    def summary(self) -> dict:
+        ok = self.cold_steps > 0
+        ratio = self.warm_steps[ok] / self.cold_steps[ok]
+        return {"cases": int(ok.sum()), "median_step_ratio": float(np.median(ratio)),
+                "mean_step_ratio": float(np.mean(ratio)), "fallbacks": int(self.fallback.sum()),
+                "cold_seconds": round(self.cold_seconds, 3),
+                "warm_seconds": round(self.warm_seconds, 3),
+                "wall_ratio": round(self.warm_seconds / self.cold_seconds, 3),
+                "inference_seconds": round(self.inference_seconds, 3)}
+
+
+def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
+            dev: str = "gpu") -> Comparison:
+    """Cold start vs learned warm start on the same cases, both judged by the plain solver.
+
+    Both runs stop when every case is below tol.  A warm start whose first residual is
+    worse than the cold start's is rejected and that case is re-run cold (its cost counts
+    the wasted warm steps)."""
+    cold, _ = solve(cases, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
+                    keep_fields=False)
+    t0 = time.perf_counter()
+    pred = net.predict(features(cases))
+    init = [from_targets(c, p) for c, p in zip(cases, pred)]
+    t_inf = time.perf_counter() - t0
+    warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
+                    dev=dev, keep_fields=False)
+    fallback = warm.first_residual > cold.first_residual
+    steps = warm.steps.copy()
+    secs = t_inf + warm.seconds
+    if fallback.any():
+        idx = np.flatnonzero(fallback)
+        redo, _ = solve([cases[i] for i in idx], tol=tol, check_every=check_every,
+                        max_steps=max_steps, dev=dev, keep_fields=False)
+        for j, i in enumerate(idx):
+            steps[i] = check_every + redo.steps[j] if redo.steps[j] >= 0 else -1
+        secs += redo.seconds
+    return Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf)

@@ -138,6 +138,13 @@ int bolt_lattice_sync(bolt_lattice* lat);
 /* Double-precision reductions (see BOLT_REDUCE_*); out receives 1 or D values. */
 int bolt_lattice_reduce(bolt_lattice* lat, int what, int tracer_id, double* out);
 
+/* Steady-state residual in double precision (no counterpart in the reference):
+ *   r = max_x |u - u_prev| / (steps_between * max_x |u|)  over non-wall cells,
+ * u_prev being the velocity saved by the previous call (which this call replaces).  The
+ * first call only saves the snapshot and returns +inf.  Reads the velocity of the last
+ * completed bolt_lattice_step. */
+int bolt_lattice_residual(bolt_lattice* lat, uint64_t steps_between, double* out);
+
 /* Device bytes held by the lattice's buffers. */
 int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);
 
@@ -150,7 +157,8 @@ int bolt_lattice_set_block(bolt_lattice* lat, int block);
  * D2Q9 fluid (counts[2k], counts[2k+1]; omega_f[2k] = w+, omega_f[2k+1] = w-, ignored for
  * BGK) with one D2Q5 tracer (omega_g[k]) coupled through acc = g * (alpha * (C - c0))
  * (alpha[k], c0[k], g[2k..2k+1]): the reference's set_boussinesq_acc_2d + update_d2q9_* +
- * update_d2q5_bgk per step (simulation.rs:353-432).  Fields are the lattice ones
+ * update_d2q5_bgk per step (simulation.rs:353-432).  With omega_g == NULL the instances are
+ * fluid only (alpha, c0 and g are then ignored).  Fields are the lattice ones
  * (BOLT_FIELD_*; the tracer is implicit).  Every instance evolves exactly as it would alone. */
 typedef struct bolt_batch bolt_batch;
 int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision,
@@ -167,11 +175,39 @@ int bolt_batch_step(bolt_batch* b, uint64_t n);
 int bolt_batch_step_timed(bolt_batch* b, uint64_t n, double* ms);
 int bolt_batch_reduce(bolt_batch* b, int inst, int what, double* out);
 int bolt_batch_cells(const bolt_batch* b, uint64_t* out);
+/* bolt_lattice_residual of one instance */
+int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
 /* autotuner: times the candidate shapes on this batch (state restored afterwards), keeps
  * the fastest and writes a per-candidate report into report[len]. */
 int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
+
+/* ---- warm-start network (no counterpart in the reference) ----------------------------
+ * A small U-Net predicting (rho, u) of a steady channel flow from per-cell features, trained
+ * and evaluated with this library's bf16 MFMA kernels (float accumulation, float master
+ * weights, Adam).  Host arrays are channels-last: inputs [n][nx][ny][4], outputs
+ * [n][nx][ny][3]; nx and ny multiples of 8.  Parameters are flattened per layer as
+ * W[cout][taps*cin] (tap = kx*3 + ky for 3x3 layers) followed by b[cout]. */
+typedef struct bolt_nn bolt_nn;
+int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, uint64_t seed,
+                   bolt_nn** out);
+int bolt_nn_destroy(bolt_nn* net);
+int bolt_nn_param_count(const bolt_nn* net, uint64_t* out);
+int bolt_nn_get_params(bolt_nn* net, float* out, uint64_t n);
+int bolt_nn_set_params(bolt_nn* net, const float* in, uint64_t n);
+/* one batch (exactly the batch size): mean squared error and its parameter gradients */
+int bolt_nn_loss_grad(bolt_nn* net, const float* x, const float* y, double* loss,
+                      float* grads, uint64_t n);
+/* uploads n training pairs to the device */
+int bolt_nn_set_data(bolt_nn* net, int n, const float* x, const float* y);
+/* minibatch Adam with cosine learning-rate decay; losses[epochs] receives the epoch means */
+int bolt_nn_train(bolt_nn* net, int epochs, float lr, uint64_t seed, float* losses);
+int bolt_nn_predict(bolt_nn* net, int n, const float* x, float* y);
+/* test hook: C[M][N] = A.B^T through the network's GEMM (a_form 0: A[m][k], 1: A[k][m];
+ * b_form 0: B[n][k], 1: B[k][n]); k_split > 0 splits K with atomic accumulation */
+int bolt_nn_gemm_check(bolt_ctx* ctx, int a_form, int b_form, int M, int N, int K,
+                       const float* a, const float* b, float* c, int k_split);
 
 #ifdef __cplusplus
 }

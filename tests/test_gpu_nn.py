@@ -1,0 +1,125 @@
+"""The warm-start network's kernels against CPU references.
+
+* the bf16 MFMA GEMM (every operand form, split-K with atomic accumulation) against numpy
+  float64 on the same bf16-rounded inputs;
+* the whole network's loss and parameter gradients against an independent torch float32
+  CPU autograd model built from the same parameters (tolerances cover the bf16 rounding of
+  activations and weights between layers);
+* training lowers the loss on a tiny fixed problem.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("a_form,b_form", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("shape,k_split", [((200, 70, 100), 0), ((64, 64, 64), 0),
+                                           ((37, 129, 300), 64), ((3, 32, 1000), 96)])
+def test_gemm_matches_numpy(a_form, b_form, shape, k_split):
+    import boltzmann_amd
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7 + N)
+    A = bf16_round(rng.standard_normal((M, K)))
+    B = bf16_round(rng.standard_normal((N, K)))
+    a = A if a_form == 0 else np.ascontiguousarray(A.T)
+    b = B if b_form == 0 else np.ascontiguousarray(B.T)
+    C = boltzmann_amd.gemm_check("gpu", a_form, b_form, a, b, k_split)
+    ref = A.astype(np.float64) @ B.astype(np.float64).T
+    np.testing.assert_allclose(C, ref, rtol=1e-5, atol=1e-4 * np.sqrt(K))
+
+
+def torch_forward(params, x, w):
+    """The network of bolt_nn.hip in torch (NCHW, float32)."""
+    import torch
+    import torch.nn.functional as F
+    spec = [(16, w, 9), (w, w, 9), (w, 2 * w, 9), (2 * w, 2 * w, 9), (2 * w, 4 * w, 9),
+            (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (8 * w, 2 * w, 9),
+            (4 * w, w, 9), (2 * w, w, 9), (w, 3, 1)]
+    Ws, bs, off = [], [], 0
+    for cin, cout, taps in spec:
+        Wf = params[off:off + cout * taps * cin]
+        off += cout * taps * cin
+        b = params[off:off + cout]
+        off += cout
+        k = 3 if taps == 9 else 1
+        # ours: W[co][tap*cin + ci], tap = kx*3 + ky  ->  torch [co][ci][kh=kx][kw=ky]
+        Wt = Wf.reshape(cout, taps, cin).transpose(0, 2, 1).reshape(cout, cin, k, k)
+        Ws.append(torch.tensor(np.ascontiguousarray(Wt), requires_grad=True))
+        bs.append(torch.tensor(np.ascontiguousarray(b), requires_grad=True))
+    assert off == params.size
+    xt = np.zeros((x.shape[0], 16) + x.shape[1:3], np.float32)
+    xt[:, :4] = x.transpose(0, 3, 1, 2)
+    X = torch.tensor(xt)
+
+    def c(i, t):
+        y = F.conv2d(t, Ws[i], bs[i], padding=1 if i < 11 else 0)
+        return F.relu(y) if i < 11 else y
+
+    up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")
+    pool = lambda t: F.avg_pool2d(t, 2)
+    e1b = c(1, c(0, X))
+    e2b = c(3, c(2, pool(e1b)))
+    e3b = c(5, c(4, pool(e2b)))
+    bb = c(7, c(6, pool(e3b)))
+    u3 = c(8, torch.cat([up(bb), e3b], 1))
+    u2 = c(9, torch.cat([up(u3), e2b], 1))
+    u1 = c(10, torch.cat([up(u2), e1b], 1))
+    out = c(11, u1)
+    return out, Ws, bs, spec
+
+
+def test_network_loss_and_gradients_match_torch():
+    import torch
+    import boltzmann_amd
+    nx, ny, B, w = 32, 16, 2, 16
+    net = boltzmann_amd.Network("gpu", nx, ny, B, width=w, seed=3)
+    params = net.get_params()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((B, nx, ny, 4)).astype(np.float32)
+    y = rng.standard_normal((B, nx, ny, 3)).astype(np.float32)
+    loss, grads = net.loss_grad(x, y)
+
+    out, Ws, bs, spec = torch_forward(params, x, w)
+    tl = torch.mean((out - torch.tensor(y.transpose(0, 3, 1, 2))) ** 2)
+    tl.backward()
+    assert abs(loss - tl.item()) / tl.item() < 2e-2, (loss, tl.item())
+    off = 0
+    for i, (cin, cout, taps) in enumerate(spec):
+        k = 3 if taps == 9 else 1
+        gW = grads[off:off + cout * taps * cin].reshape(cout, taps, cin).transpose(0, 2, 1)
+        gW = gW.reshape(cout, cin, k, k)
+        off += cout * taps * cin
+        gb = grads[off:off + cout]
+        off += cout
+        for ours, ref in ((gW, Ws[i].grad.numpy()), (gb, bs[i].grad.numpy())):
+            ref = ref.ravel().astype(np.float64)
+            ours = ours.ravel().astype(np.float64)
+            cos = ours @ ref / (np.linalg.norm(ours) * np.linalg.norm(ref) + 1e-30)
+            rel = np.linalg.norm(ours - ref) / (np.linalg.norm(ref) + 1e-30)
+            assert cos > 0.99 and rel < 0.1, (i, cos, rel)
+
+
+def test_training_reduces_loss():
+    import boltzmann_amd
+    nx, ny, B = 32, 16, 4
+    net = boltzmann_amd.Network("gpu", nx, ny, B, width=16, seed=1)
+    rng = np.random.default_rng(1)
+    n = 16
+    x = rng.standard_normal((n, nx, ny, 4)).astype(np.float32)
+    # a smooth target that depends on the input: a learnable map
+    y = np.stack([np.tanh(x[..., 0]), 0.5 * x[..., 1], x[..., 0] * x[..., 2] * 0.1], -1).astype(np.float32)
+    net.set_data(x, y)
+    losses = net.train(30, lr=2e-3, seed=0)
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], losses
+    pred = net.predict(x[:5])
+    assert pred.shape == (5, nx, ny, 3) and np.isfinite(pred).all()
