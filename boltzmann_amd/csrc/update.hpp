@@ -117,15 +117,37 @@ struct Nbr<3> {
 };
 
 // ---- AA loads and stores ---------------------------------------------------------------
+// Populations are touched once per step, so they are loaded with the streaming
+// (non-temporal) hint: measured on MI355X, D3Q27 TRT 256^3, it lifts FP16-shifted storage
+// from 42.1k to 45.7k MLUPS and float storage by 2 %.  Streaming stores measured slower
+// and are off unless BOLT_NT_STORE is defined; BOLT_NO_NT_LOAD turns the load hint off.
+template <class T>
+__device__ inline T pop_load(const void* plane, uint32_t cell) {
+#ifndef BOLT_NO_NT_LOAD
+    return __builtin_nontemporal_load(&elem<T>(plane, cell));
+#else
+    return elem<T>(plane, cell);
+#endif
+}
+
+template <class T>
+__device__ inline void pop_store(void* plane, uint32_t cell, T v) {
+#ifdef BOLT_NT_STORE
+    __builtin_nontemporal_store(v, &elem<T>(plane, cell));
+#else
+    elem<T>(plane, cell) = v;
+#endif
+}
+
 template <class V, bool EVEN, class T, class NB>
 __device__ inline void aa_load_raw(void* const* planes, const NB& nb, uint32_t cell,
                                    T (&v)[V::q]) {
 #pragma unroll
     for (int i = 0; i < V::q; ++i) {
         if (EVEN)
-            v[i] = elem<T>((const void*)planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)));
+            v[i] = pop_load<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)));
         else
-            v[i] = elem<T>((const void*)planes[V::opp(i)], cell);
+            v[i] = pop_load<T>(planes[V::opp(i)], cell);
     }
 }
 
@@ -135,9 +157,9 @@ __device__ inline void aa_store_raw(void* const* planes, const NB& nb, uint32_t 
 #pragma unroll
     for (int i = 0; i < V::q; ++i) {
         if (EVEN)
-            elem<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2))) = v[V::opp(i)];
+            pop_store<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)), v[V::opp(i)]);
         else
-            elem<T>(planes[i], cell) = v[i];
+            pop_store<T>(planes[i], cell, v[i]);
     }
 }
 

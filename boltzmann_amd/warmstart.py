@@ -113,6 +113,8 @@ class SolveResult:
     residual: np.ndarray     # last residual per case
     seconds: float
     first_residual: np.ndarray
+    history: np.ndarray = None   # [checks, cases] residual at every check
+    check_every: int = 0
 
 
 def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
@@ -132,6 +134,7 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
     steps = np.full(n, -1, np.int64)
     last = np.full(n, np.inf)
     first = np.full(n, np.nan)
+    hist = []
     t0 = time.perf_counter()
     done = 0
     while done < max_steps:
@@ -147,10 +150,11 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
                 first[k] = r
             if steps[k] < 0 and r <= tol:
                 steps[k] = done
+        hist.append(last.copy())
         if stop_when_done and (steps >= 0).all():
             break
     secs = time.perf_counter() - t0
-    res = SolveResult(steps, last, secs, first)
+    res = SolveResult(steps, last, secs, first, np.array(hist), check_every)
     if not keep_fields:
         return res, None
     fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
@@ -232,12 +236,13 @@ class Comparison:
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
-            dev: str = "gpu") -> Comparison:
+            judge_at: int = 1000, dev: str = "gpu") -> Comparison:
     """Cold start vs learned warm start on the same cases, both judged by the plain solver.
 
-    Both runs stop when every case is below tol.  A warm start whose first residual is
-    worse than the cold start's is rejected and that case is re-run cold (its cost counts
-    the wasted warm steps)."""
+    Both runs stop when every case is below tol.  After judge_at steps the warm run's
+    residual is compared with the cold run's at the same step count; a warm start that is
+    worse is rejected and that case is re-run cold, its cost counting the wasted warm
+    steps."""
     cold, _ = solve(cases, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
                     keep_fields=False)
     t0 = time.perf_counter()
@@ -246,14 +251,20 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
     t_inf = time.perf_counter() - t0
     warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
                     dev=dev, keep_fields=False)
-    fallback = warm.first_residual > cold.first_residual
+    j = max(0, judge_at // check_every - 1)
+    rw = warm.history[min(j, len(warm.history) - 1)]
+    rc = cold.history[min(j, len(cold.history) - 1)]
+    # a case already converged by the judging step is never rejected
+    fallback = (rw > rc) & ~((warm.steps >= 0) & (warm.steps <= (j + 1) * check_every))
     steps = warm.steps.copy()
     secs = t_inf + warm.seconds
     if fallback.any():
         idx = np.flatnonzero(fallback)
         redo, _ = solve([cases[i] for i in idx], tol=tol, check_every=check_every,
                         max_steps=max_steps, dev=dev, keep_fields=False)
-        for j, i in enumerate(idx):
-            steps[i] = check_every + redo.steps[j] if redo.steps[j] >= 0 else -1
+        for k, i in enumerate(idx):
+            steps[i] = (j + 1) * check_every + redo.steps[k] if redo.steps[k] >= 0 else -1
         secs += redo.seconds
-    return Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf)
+    cmp = Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf)
+    cmp.warm_raw_steps = warm.steps
+    return cmp

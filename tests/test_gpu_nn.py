@@ -37,10 +37,29 @@ def test_gemm_matches_numpy(a_form, b_form, shape, k_split):
     np.testing.assert_allclose(C, ref, rtol=1e-5, atol=1e-4 * np.sqrt(K))
 
 
-def torch_forward(params, x, w):
-    """The network of bolt_nn.hip in torch (NCHW, float32)."""
+def _bf16(t):
+    import torch
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def torch_forward(params, x, w, emulate=True):
+    """The network of bolt_nn.hip in torch (NCHW, float32).  With emulate, values are rounded
+    to bf16 where the GPU kernels store bf16: weights and the input, every layer's stored
+    activation, and (in the backward pass) every pre-activation gradient."""
     import torch
     import torch.nn.functional as F
+
+    class GradRound(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, t):
+            return t
+
+        @staticmethod
+        def backward(ctx, g):
+            return _bf16(g)
+
+    R = _bf16 if emulate else (lambda t: t)
+    G = GradRound.apply if emulate else (lambda t: t)
     spec = [(16, w, 9), (w, w, 9), (w, 2 * w, 9), (2 * w, 2 * w, 9), (2 * w, 4 * w, 9),
             (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (8 * w, 2 * w, 9),
             (4 * w, w, 9), (2 * w, w, 9), (w, 3, 1)]
@@ -58,14 +77,14 @@ def torch_forward(params, x, w):
     assert off == params.size
     xt = np.zeros((x.shape[0], 16) + x.shape[1:3], np.float32)
     xt[:, :4] = x.transpose(0, 3, 1, 2)
-    X = torch.tensor(xt)
+    X = R(torch.tensor(xt))
 
     def c(i, t):
-        y = F.conv2d(t, Ws[i], bs[i], padding=1 if i < 11 else 0)
-        return F.relu(y) if i < 11 else y
+        y = G(F.conv2d(t, R(Ws[i]), bs[i], padding=1 if i < 11 else 0))
+        return R(F.relu(y)) if i < 11 else y
 
     up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")
-    pool = lambda t: F.avg_pool2d(t, 2)
+    pool = lambda t: R(F.avg_pool2d(t, 2))
     e1b = c(1, c(0, X))
     e2b = c(3, c(2, pool(e1b)))
     e3b = c(5, c(4, pool(e2b)))
@@ -91,8 +110,9 @@ def test_network_loss_and_gradients_match_torch():
     out, Ws, bs, spec = torch_forward(params, x, w)
     tl = torch.mean((out - torch.tensor(y.transpose(0, 3, 1, 2))) ** 2)
     tl.backward()
-    assert abs(loss - tl.item()) / tl.item() < 2e-2, (loss, tl.item())
+    assert abs(loss - tl.item()) / tl.item() < 2e-3, (loss, tl.item())
     off = 0
+    report = []
     for i, (cin, cout, taps) in enumerate(spec):
         k = 3 if taps == 9 else 1
         gW = grads[off:off + cout * taps * cin].reshape(cout, taps, cin).transpose(0, 2, 1)
@@ -105,7 +125,10 @@ def test_network_loss_and_gradients_match_torch():
             ours = ours.ravel().astype(np.float64)
             cos = ours @ ref / (np.linalg.norm(ours) * np.linalg.norm(ref) + 1e-30)
             rel = np.linalg.norm(ours - ref) / (np.linalg.norm(ref) + 1e-30)
-            assert cos > 0.99 and rel < 0.1, (i, cos, rel)
+            report.append((i, round(cos, 5), round(rel, 4)))
+    print("per-layer (cos, rel) of W then b:", report, flush=True)
+    for i, cos, rel in report:
+        assert cos > 0.999 and rel < 0.05, report
 
 
 def test_training_reduces_loss():
