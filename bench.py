@@ -185,6 +185,40 @@ def cylinder(steps_marks=(1000, 5000, 20000)):
     return out
 
 
+def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
+                      width: int = 32, max_steps: int = 60000):
+    """Learned warm start against the plain solver's time to steady state.
+
+    D2Q9 TRT channels of 512 x 256 cells with 1-8 seeded obstacles (radius / half-size
+    8-32), u_in in [0.02, 0.05], Re in [10, 100].  n_train cases (seeds 0..) are solved to
+    the FP64 residual 1e-6 in batches; those that converge within max_steps train the
+    network; n_test held-out cases (seeds 10^6..) run cold and warm to the same residual."""
+    from boltzmann_amd import warmstart as ws
+
+    def log(msg):
+        print(f"[bench] warm start: {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    cases, fields, steps = ws.generate(n_train, seed0=0, batch=125, max_steps=max_steps,
+                                       check_every=2000, log=log)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=width, lr=1e-3,
+                           seed=0, log=log)
+    t_train = time.perf_counter() - t0
+    del fields
+    held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
+    cmp = ws.compare(net, held, check_every=500, max_steps=2 * max_steps)
+    s = cmp.summary()
+    log(f"held-out: {s}")
+    return {"train_cases": len(cases), "train_requested": n_train,
+            "train_steps_median": int(np.median(steps)) if len(steps) else None,
+            "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
+            "epochs": epochs, "width": width, "params": int(net.param_count),
+            "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
+            "held_out": s}
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of nbytes (hipMemcpy): a kernel of known traffic for
     scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -213,6 +247,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rb-steps", type=int, default=50000)
+    ap.add_argument("--warmstart", action="store_true",
+                    help="add the learned warm start vs cold start on 512x256 channels "
+                         "(about 15 minutes)")
+    ap.add_argument("--ws-train", type=int, default=2000)
+    ap.add_argument("--ws-test", type=int, default=100)
+    ap.add_argument("--ws-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -249,6 +289,9 @@ def main() -> int:
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
+    if args.warmstart:
+        variants["warmstart_channel_512x256"] = warmstart_channel(
+            args.ws_train, args.ws_test, args.ws_epochs)
 
     achieved = head["mlups"] * 1e6 * B_ALG["f32"] / 1e9  # GB/s
     line = {

@@ -466,7 +466,15 @@ int bolt_nn_set_data(bolt_nn* N, int n, const float* x, const float* y) {
 
 // Minibatch Adam over the uploaded dataset; losses[e] receives epoch e's mean batch loss.
 int bolt_nn_train(bolt_nn* N, int epochs, float lr, uint64_t seed, float* losses) {
-    if (!N || epochs < 1) return fail(BOLT_EINVAL, "bad arguments");
+    return bolt_nn_train_part(N, 0, epochs, epochs, lr, seed, losses);
+}
+
+// Epochs [first, first + epochs) of a run of total epochs: the same shuffles and cosine
+// learning rates as those epochs of one bolt_nn_train(total) call.
+int bolt_nn_train_part(bolt_nn* N, int first, int epochs, int total, float lr, uint64_t seed,
+                       float* losses) {
+    if (!N || epochs < 1 || first < 0 || first + epochs > total)
+        return fail(BOLT_EINVAL, "bad arguments");
     if (N->data_n < N->batch) return fail(BOLT_ESTATE, "dataset smaller than one batch (bolt_nn_set_data)");
     HIP_TRY(hipSetDevice(N->ctx->device));
     hipStream_t s = N->ctx->stream;
@@ -475,12 +483,13 @@ int bolt_nn_train(bolt_nn* N, int epochs, float lr, uint64_t seed, float* losses
     std::iota(order.begin(), order.end(), 0);
     std::mt19937_64 rng(seed);
     const int per_epoch = N->data_n / N->batch;
-    for (int e = 0; e < epochs; ++e) {
+    for (int e = 0; e < first; ++e) std::shuffle(order.begin(), order.end(), rng);
+    for (int e = first; e < first + epochs; ++e) {
         std::shuffle(order.begin(), order.end(), rng);
         // cosine decay of the learning rate over the run
         double tot = 0;
         for (int b = 0; b < per_epoch; ++b) {
-            const float frac = (float)(e * per_epoch + b) / (float)(epochs * per_epoch);
+            const float frac = (float)((long)e * per_epoch + b) / (float)((long)total * per_epoch);
             const float lr_t = lr * 0.5f * (1.f + cosf(3.14159265f * frac));
             for (int k = 0; k < N->batch; ++k) {
                 const int idx = order[b * N->batch + k];
@@ -500,7 +509,7 @@ int bolt_nn_train(bolt_nn* N, int epochs, float lr, uint64_t seed, float* losses
             if (rc) return rc;
             tot += l;
         }
-        if (losses) losses[e] = (float)(tot / per_epoch);
+        if (losses) losses[e - first] = (float)(tot / per_epoch);
     }
     HIP_TRY(hipStreamSynchronize(s));
     return BOLT_OK;

@@ -18,6 +18,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -732,11 +733,14 @@ PYBIND11_MODULE(_boltzmann, m) {
                 throw BoltError("set_data: x must be [n, nx, ny, 4] and y [n, nx, ny, 3]");
             check(bolt_nn_set_data(c.net, (int)cnt, x.data(), y.data()), "set_data");
         })
-        .def("train", [](PyNet& n, int epochs, float lr, uint64_t seed) {
-            py::array_t<float> losses(epochs);
-            check(bolt_nn_train(n.core->net, epochs, lr, seed, losses.mutable_data()), "train");
+        .def("train", [](PyNet& n, int epochs, float lr, uint64_t seed, int first, int total) {
+            py::array_t<float> losses(std::max(epochs, 0));
+            if (total <= 0) total = first + epochs;
+            check(bolt_nn_train_part(n.core->net, first, epochs, total, lr, seed, losses.mutable_data()),
+                  "train");
             return losses;
-        }, py::arg("epochs"), py::arg("lr") = 1e-3f, py::arg("seed") = 0)
+        }, py::arg("epochs"), py::arg("lr") = 1e-3f, py::arg("seed") = 0, py::arg("first") = 0,
+           py::arg("total") = 0)
         .def("predict", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> x) {
             NetCore& c = *n.core;
             const size_t pix = (size_t)c.nx * c.ny;

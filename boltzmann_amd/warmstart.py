@@ -203,16 +203,27 @@ def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
 
 
 def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
-          seed: int = 0, dev: str = "gpu"):
-    """Trains the network on solver-generated steady states with the library's kernels."""
+          seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1):
+    """Trains the network on solver-generated steady states with the library's kernels.
+
+    With log, training runs log_every epochs at a time (same schedule as one call) and
+    reports each chunk's last epoch loss."""
     from . import Network
     c0 = cases[0]
     net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed)
     x = features(cases)
     y = np.stack([to_targets(c, *f) for c, f in zip(cases, fields)])
     net.set_data(x, y)
-    losses = net.train(epochs, lr=lr, seed=seed)
-    return net, losses
+    del x, y
+    if log is None:
+        return net, net.train(epochs, lr=lr, seed=seed)
+    losses = []
+    t0 = time.perf_counter()
+    for e0 in range(0, epochs, log_every):
+        k = min(log_every, epochs - e0)
+        losses.extend(net.train(k, lr=lr, seed=seed, first=e0, total=epochs).tolist())
+        log(f"epoch {e0 + k}/{epochs}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return net, np.array(losses, np.float32)
 
 
 @dataclass

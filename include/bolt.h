@@ -203,6 +203,10 @@ int bolt_nn_loss_grad(bolt_nn* net, const float* x, const float* y, double* loss
 int bolt_nn_set_data(bolt_nn* net, int n, const float* x, const float* y);
 /* minibatch Adam with cosine learning-rate decay; losses[epochs] receives the epoch means */
 int bolt_nn_train(bolt_nn* net, int epochs, float lr, uint64_t seed, float* losses);
+/* epochs [first, first+epochs) of a run of `total` epochs: the same shuffles and cosine
+ * learning rates as those epochs of one bolt_nn_train(total) call (progress reporting) */
+int bolt_nn_train_part(bolt_nn* net, int first, int epochs, int total, float lr, uint64_t seed,
+                       float* losses);
 int bolt_nn_predict(bolt_nn* net, int n, const float* x, float* y);
 /* test hook: C[M][N] = A.B^T through the network's GEMM (a_form 0: A[m][k], 1: A[k][m];
  * b_form 0: B[n][k], 1: B[k][n]); k_split > 0 splits K with atomic accumulation */
