@@ -94,8 +94,9 @@ def _batch(cases, dev="gpu") -> Batch:
     return Batch(dev, [(c.nx, c.ny) for c in cases], "trt", [c.omega for c in cases])
 
 
-def load_state(b: Batch, k: int, case: Case, rho: np.ndarray, vel: np.ndarray) -> None:
-    """Writes flags and an initial (rho, u); fixed cells keep their prescribed values."""
+def load_state(b: Batch, k: int, case: Case, rho: np.ndarray, vel: np.ndarray):
+    """Writes flags and an initial (rho, u); fixed cells keep their prescribed values.
+    Returns the (rho, u) written."""
     cold_rho, cold_vel = case.cold_state()
     flags = case.flags()
     fixed = (flags & 6) != 0
@@ -105,6 +106,37 @@ def load_state(b: Batch, k: int, case: Case, rho: np.ndarray, vel: np.ndarray) -
     b.write(k, "flags", flags)
     b.write(k, "rho", rho)
     b.write(k, "vel", vel)
+    return rho, vel
+
+
+# D2Q9 velocities and weights in the reference's slot order (velocities.rs:66-83)
+_C9 = np.array([(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1)],
+               np.float64)
+_W9 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def populations(case: Case, rho: np.ndarray, vel: np.ndarray) -> np.ndarray:
+    """Pre-collision D2Q9 populations [nx, ny, 9] for a given (rho, u).
+
+    Equilibrium plus the first-order Chapman-Enskog non-equilibrium part
+    f_i^(1) = -tau+ w_i rho Q_i:S / cs^2, Q_i = c_i c_i - cs^2 I, S the strain rate from
+    central differences with wall cells at u = 0 (Skordos 1993; Mei et al. 2006).  A state
+    started this way carries its viscous stress instead of relaxing to it, which would
+    launch sound waves that the fixed inflow and outflow columns reflect back and forth."""
+    r = rho.astype(np.float64)
+    u = vel.astype(np.float64).copy()
+    u[case.mask] = 0.0
+    cu = u @ _C9.T
+    uu = (u ** 2).sum(-1)[..., None]
+    f = _W9 * r[..., None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * uu)
+    dux_dx, dux_dy = np.gradient(u[..., 0])
+    duy_dx, duy_dy = np.gradient(u[..., 1])
+    sxy = 0.5 * (dux_dy + duy_dx)
+    cx, cy = _C9[:, 0], _C9[:, 1]
+    qs = ((cx ** 2 - 1 / 3) * dux_dx[..., None] + (cy ** 2 - 1 / 3) * duy_dy[..., None]
+          + 2 * cx * cy * sxy[..., None])
+    f -= 3.0 * case.tau * _W9 * r[..., None] * qs
+    return f.astype(np.float32)
 
 
 @dataclass
@@ -118,18 +150,24 @@ class SolveResult:
 
 
 def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
-          batch: Batch | None = None, stop_when_done=True, keep_fields=True):
+          batch: Batch | None = None, stop_when_done=True, keep_fields=True, neq=False):
     """Runs every case to the residual tolerance in one batch.
 
-    init: None for the cold start, else a list of (rho, vel) per case.  Returns the
-    SolveResult and, with keep_fields, the final (rho, vel) of every case."""
+    init: None for the cold start, else a list of (rho, vel) per case; with neq the
+    initial populations carry the non-equilibrium part of populations(), else they are at
+    equilibrium.  Returns the SolveResult and, with keep_fields, the final (rho, vel) of
+    every case."""
     b = batch or _batch(cases, dev)
+    written = []
     for k, c in enumerate(cases):
         if init is None:
-            load_state(b, k, c, *c.cold_state())
+            written.append(load_state(b, k, c, *c.cold_state()))
         else:
-            load_state(b, k, c, *init[k])
+            written.append(load_state(b, k, c, *init[k]))
     b.equilibrate()
+    if neq and init is not None:
+        for k, c in enumerate(cases):
+            b.write(k, "f", populations(c, *written[k]))
     n = len(cases)
     steps = np.full(n, -1, np.int64)
     last = np.full(n, np.inf)
@@ -247,7 +285,7 @@ class Comparison:
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
-            judge_at: int = 1000, dev: str = "gpu") -> Comparison:
+            judge_at: int = 1000, dev: str = "gpu", neq: bool = False) -> Comparison:
     """Cold start vs learned warm start on the same cases, both judged by the plain solver.
 
     Both runs stop when every case is below tol.  After judge_at steps the warm run's
@@ -261,7 +299,7 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
     init = [from_targets(c, p) for c, p in zip(cases, pred)]
     t_inf = time.perf_counter() - t0
     warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
-                    dev=dev, keep_fields=False)
+                    dev=dev, keep_fields=False, neq=neq)
     j = max(0, judge_at // check_every - 1)
     rw = warm.history[min(j, len(warm.history) - 1)]
     rc = cold.history[min(j, len(cold.history) - 1)]
