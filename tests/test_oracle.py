@@ -295,3 +295,32 @@ def test_odd_iterate_rejected():
     sim = oracle.OracleSimulation((8, 8), ("bgk", 1.0))
     with pytest.raises(ValueError):
         sim.iterate(3)
+
+
+# ---- physics closed forms (tests/closed_forms.py) ---------------------------------------
+import closed_forms as cf  # noqa: E402
+
+
+@pytest.mark.parametrize("coll", ["bgk", "trt"])
+def test_shear_wave_viscosity(coll):
+    got, want = cf.shear_wave(cf.OracleBackend, coll)
+    assert abs(got / want - 1) < 2e-3, (got, want)
+
+
+@pytest.mark.parametrize("coll,magic", [("bgk", 0.25), ("trt", 0.25), ("trt", 3 / 16)])
+def test_poiseuille_with_reference_force_semantics(coll, magic):
+    # a forcing shift divided by omega- (instead of the reference's omega+) would leave
+    # TRT at tau = 0.8, magic 1/4 about 40 % off this profile
+    assert cf.poiseuille(cf.OracleBackend, coll, magic=magic) < 2e-3
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_gaussian_tracer_diffusivity(d):
+    got, want = cf.gaussian_diffusion(cf.OracleBackend, d)
+    np.testing.assert_allclose(got, want, rtol=1e-4)
+
+
+@pytest.mark.parametrize("coll", ["bgk", "trt"])
+def test_taylor_green_energy_decay(coll):
+    got, want = cf.taylor_green(cf.OracleBackend, coll, n=32, dt=200)
+    assert abs(got / want - 1) < 1e-2, (got, want)
