@@ -241,6 +241,22 @@ def test_simulation_serde():  # test_core.py:106-127
         sim1.add_boussinesq_coupling(trc1, 0.1, 0, np.array([0, 1], np.float32))
         sim1.iterate(10)
         sim1.write_checkpoint(path)
+        # the file is positional MessagePack in the layout of simulation.rs:130-176:
+        # [cells, fluid, tracers, gravity, iteration, couplings], arrays as [1, dims, data]
+        import msgpack
+        doc = msgpack.unpackb(open(path, "rb").read(), strict_map_key=False)
+        cells, fluid, tracers, gravity, iteration, couplings = doc
+        assert cells[0][0][:2] == [1, [100, 100]] and len(cells[0][0][2]) == 100 * 100
+        assert cells[1] == [1, [2], [100, 100]]
+        assert fluid[0][0][1] == [100, 100, 9] and fluid[2][0][1] == [100, 100, 2]
+        assert fluid[3] is not None                      # acceleration buffer exists
+        assert fluid[4][0][1] == [9] and fluid[4][1][1] == [9, 2]
+        assert list(fluid[5]) == ["BGK"] and abs(fluid[5]["BGK"] - 1 / 0.51) < 1e-6
+        assert list(tracers) == ["tracer"] and tracers["tracer"][2][0][1] == [5]
+        assert gravity is None and iteration == 10
+        assert couplings[0][0] == "tracer" and couplings[0][3][0] == [1, [2], [0.0, 1.0]]
+        np.testing.assert_array_equal(np.array(fluid[1][0][2], np.float32).reshape(100, 100),
+                                      sim1.fluid.rho)
         sim2 = bolt.Simulation.load_checkpoint("gpu", path)
         trc2 = sim2.get_tracer("tracer")
         assert np.allclose(sim1.fluid.rho, sim2.fluid.rho)
