@@ -214,7 +214,7 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
     return {"train_cases": len(cases), "train_requested": n_train,
             "train_steps_median": int(np.median(steps)) if len(steps) else None,
             "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
-            "epochs": epochs, "width": width, "params": int(net.param_count),
+            "epochs": epochs, "width": width, "params": int(net.net.param_count),
             "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
             "held_out": s}
 

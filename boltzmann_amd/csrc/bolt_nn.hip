@@ -1,19 +1,23 @@
 // The warm-start network: a small U-Net trained and run with this library's own kernels
 // (nn_kernels.hpp: bf16 MFMA GEMM, im2col/col2im, element-wise layers).
 //
-// Input  [batch][nx][ny][16]  bf16 (the caller's 4 feature channels, zero-padded to 16)
+// Input  [batch][nx][ny][16]  bf16 (the caller's 8 feature channels, zero-padded to 16)
 // Output [batch][nx][ny][3]   float
 //
+// A U-Net of depth D (2x2 poolings; D = 3 below), channels ch[l] = w * min(2^l, 4) at
+// level l and decoder channels dch[l] = ch[max(l - 1, 0)]:
 //   level 0 (nx, ny):     e1a = conv(x, 16->w)          e1b = conv(e1a, w->w)
 //   level 1 (nx/2, ny/2): e2a = conv(pool e1b, w->2w)   e2b = conv(e2a, 2w->2w)
 //   level 2:              e3a = conv(pool e2b, 2w->4w)  e3b = conv(e3a, 4w->4w)
-//   level 3:              ba  = conv(pool e3b, 4w->4w)  bb  = conv(ba, 4w->4w)
+//   level 3 (bottleneck): ba  = conv(pool e3b, 4w->4w)  bb  = conv(ba, 4w->4w)
 //   level 2:  u3 = conv([up bb | e3b], 8w->2w)
 //   level 1:  u2 = conv([up u3 | e2b], 4w->w)
 //   level 0:  u1 = conv([up u2 | e1b], 2w->w)          out = conv1x1(u1, w->3)
 // 3x3 convolutions with padding 1 and ReLU, 2x2 average pooling, nearest upsampling.
-// Parameters are float master weights (layer order above, each W[cout][taps*cin] then
-// b[cout], tap = kx*3 + ky), with a bf16 copy for the GEMMs; Adam updates both.
+// Parameters are float master weights (layer order: the encoder pairs level by level, the
+// bottleneck pair, the decoder convolutions from the deepest level up, the 1x1 output;
+// each W[cout][taps*cin] then b[cout], tap = kx*3 + ky), with a bf16 copy for the GEMMs;
+// Adam updates both.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -34,7 +38,7 @@ using namespace nnk;
 namespace {
 
 constexpr int kInC = 16;   // padded input channels
-constexpr int kFeat = 4;   // caller's feature channels
+constexpr int kFeat = 8;   // caller's feature channels
 constexpr int kOutC = 3;
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -54,8 +58,9 @@ struct Buf {
 
 struct bolt_nn {
     bolt_ctx* ctx = nullptr;
-    int nx = 0, ny = 0, batch = 0, w = 32;
+    int nx = 0, ny = 0, batch = 0, w = 32, depth = 3;
     std::vector<Conv> conv;
+    std::vector<int> ch, dch;   // encoder / decoder channels per level (0..depth)
     size_t nparam = 0;
     float* W = nullptr;    // float master parameters
     u16* Wb = nullptr;     // bf16 copy
@@ -64,12 +69,14 @@ struct bolt_nn {
     float* M2 = nullptr;
     int adam_t = 0;
     std::vector<Buf> bufs;
-    // activations (bf16)
-    u16 *x0, *e1a, *cat1, *p1, *e2a, *cat2, *p2, *e3a, *cat3, *p3, *ba, *bb, *u3, *u2, *u1;
+    // activations (bf16), per level: ea = first conv of the level, cat = [up | second conv]
+    // (levels < depth), pin = pooled input (levels >= 1; pin[0] = x0), u = decoder output
+    u16 *x0 = nullptr, *bb = nullptr;
+    std::vector<u16*> ea, cat, pin, u;
     float* out;
     // gradients (float)
-    float *dout, *du1, *dcat1, *du2, *dcat2, *du3, *dcat3, *dbb, *dba, *dp3, *de3a, *dp2, *de2a,
-        *dp1, *de1a;
+    float *dout = nullptr, *dbb = nullptr;
+    std::vector<float*> du, dcat, dpin, dea;
     u16* dz;       // bf16 pre-activation gradient of the layer being processed
     u16* col;      // im2col scratch
     float* dcol;   // input-gradient columns
@@ -79,6 +86,13 @@ struct bolt_nn {
     u16* data_x = nullptr;
     float* data_y = nullptr;
     int data_n = 0;
+
+    // layer indices: level l's pair (the bottleneck is level depth), decoder, output
+    int la(int l) const { return 2 * l; }
+    int lb(int l) const { return 2 * l + 1; }
+    int lu(int l) const { return 2 * depth + 2 + (depth - 1 - l); }
+    int lout() const { return 3 * depth + 2; }
+    int catc(int l) const { return dch[l + 1] + ch[l]; }
 };
 
 static int nn_alloc(bolt_nn* N, void** p, size_t bytes) {
@@ -176,64 +190,63 @@ static void up_bwd(bolt_nn* N, const float* dY, int ldy, int yoff, float* dX, in
 }
 
 static void forward(bolt_nn* N) {
-    const int w = N->w, X0 = N->nx, Y0 = N->ny;
-    const int X1 = X0 / 2, Y1 = Y0 / 2, X2 = X1 / 2, Y2 = Y1 / 2, X3 = X2 / 2, Y3 = Y2 / 2;
+    const int D = N->depth;
     auto& C = N->conv;
-    conv_fwd(N, C[0], N->x0, X0, Y0, N->e1a, w, 0);
-    conv_fwd(N, C[1], N->e1a, X0, Y0, N->cat1, 2 * w, w);
-    pool(N, N->cat1, 2 * w, w, N->p1, X0, Y0, w);
-    conv_fwd(N, C[2], N->p1, X1, Y1, N->e2a, 2 * w, 0);
-    conv_fwd(N, C[3], N->e2a, X1, Y1, N->cat2, 4 * w, 2 * w);
-    pool(N, N->cat2, 4 * w, 2 * w, N->p2, X1, Y1, 2 * w);
-    conv_fwd(N, C[4], N->p2, X2, Y2, N->e3a, 4 * w, 0);
-    conv_fwd(N, C[5], N->e3a, X2, Y2, N->cat3, 8 * w, 4 * w);
-    pool(N, N->cat3, 8 * w, 4 * w, N->p3, X2, Y2, 4 * w);
-    conv_fwd(N, C[6], N->p3, X3, Y3, N->ba, 4 * w, 0);
-    conv_fwd(N, C[7], N->ba, X3, Y3, N->bb, 4 * w, 0);
-    up(N, N->bb, N->cat3, 8 * w, 0, X2, Y2, 4 * w);
-    conv_fwd(N, C[8], N->cat3, X2, Y2, N->u3, 2 * w, 0);
-    up(N, N->u3, N->cat2, 4 * w, 0, X1, Y1, 2 * w);
-    conv_fwd(N, C[9], N->cat2, X1, Y1, N->u2, w, 0);
-    up(N, N->u2, N->cat1, 2 * w, 0, X0, Y0, w);
-    conv_fwd(N, C[10], N->cat1, X0, Y0, N->u1, w, 0);
-    const Conv& O = C[11];
-    const long long P = (long long)N->batch * X0 * Y0;
-    gemm(N, kMK, kNK, kStoreF32, N->u1, w, N->Wb + O.w_off, w, N->out, kOutC, 0, (int)P, kOutC, w,
-         N->W + O.b_off, false);
+    for (int l = 0; l < D; ++l) {
+        const int X = N->nx >> l, Y = N->ny >> l, cc = N->catc(l);
+        conv_fwd(N, C[N->la(l)], N->pin[l], X, Y, N->ea[l], N->ch[l], 0);
+        conv_fwd(N, C[N->lb(l)], N->ea[l], X, Y, N->cat[l], cc, N->dch[l + 1]);
+        pool(N, N->cat[l], cc, N->dch[l + 1], N->pin[l + 1], X, Y, N->ch[l]);
+    }
+    const int XD = N->nx >> D, YD = N->ny >> D;
+    conv_fwd(N, C[N->la(D)], N->pin[D], XD, YD, N->ea[D], N->ch[D], 0);
+    conv_fwd(N, C[N->lb(D)], N->ea[D], XD, YD, N->bb, N->ch[D], 0);
+    const u16* prev = N->bb;
+    for (int l = D - 1; l >= 0; --l) {
+        const int X = N->nx >> l, Y = N->ny >> l;
+        up(N, prev, N->cat[l], N->catc(l), 0, X, Y, N->dch[l + 1]);
+        conv_fwd(N, C[N->lu(l)], N->cat[l], X, Y, N->u[l], N->dch[l], 0);
+        prev = N->u[l];
+    }
+    const Conv& O = C[N->lout()];
+    const long long P = (long long)N->batch * N->nx * N->ny;
+    gemm(N, kMK, kNK, kStoreF32, N->u[0], O.cin, N->Wb + O.w_off, O.cin, N->out, kOutC, 0, (int)P, kOutC,
+         O.cin, N->W + O.b_off, false);
 }
 
 static void backward(bolt_nn* N) {
-    const int w = N->w, X0 = N->nx, Y0 = N->ny;
-    const int X1 = X0 / 2, Y1 = Y0 / 2, X2 = X1 / 2, Y2 = Y1 / 2, X3 = X2 / 2, Y3 = Y2 / 2;
+    const int D = N->depth;
     auto& C = N->conv;
     hipStream_t s = N->ctx->stream;
-    const long long P0 = (long long)N->batch * X0 * Y0;
+    const long long P0 = (long long)N->batch * N->nx * N->ny;
     // output 1x1 conv
-    const Conv& O = C[11];
+    const Conv& O = C[N->lout()];
+    const int w0 = O.cin;
     hipLaunchKernelGGL(relu_grad, dim3(nblk(P0 * kOutC)), dim3(256), 0, s, N->dout, kOutC, 0, (const u16*)nullptr,
                        0, 0, N->dz, P0, kOutC, 0);
     hipLaunchKernelGGL(bias_grad, dim3(kOutC), dim3(256), 0, s, N->dz, P0, kOutC, N->G + O.b_off);
-    gemm(N, kKM, kKN, kAtomicF32, N->dz, kOutC, N->u1, w, N->G + O.w_off, w, 0, kOutC, w, (int)P0, nullptr,
+    gemm(N, kKM, kKN, kAtomicF32, N->dz, kOutC, N->u[0], w0, N->G + O.w_off, w0, 0, kOutC, w0, (int)P0, nullptr,
          false, 4096);
-    gemm(N, kMK, kKN, kStoreF32, N->dz, kOutC, N->Wb + O.w_off, w, N->du1, w, 0, (int)P0, w, kOutC, nullptr,
+    gemm(N, kMK, kKN, kStoreF32, N->dz, kOutC, N->Wb + O.w_off, w0, N->du[0], w0, 0, (int)P0, w0, kOutC, nullptr,
          false);
-    conv_bwd(N, C[10], N->cat1, X0, Y0, N->u1, w, 0, N->du1, w, 0, N->dcat1);
-    up_bwd(N, N->dcat1, 2 * w, 0, N->du2, X0, Y0, w);
-    conv_bwd(N, C[9], N->cat2, X1, Y1, N->u2, w, 0, N->du2, w, 0, N->dcat2);
-    up_bwd(N, N->dcat2, 4 * w, 0, N->du3, X1, Y1, 2 * w);
-    conv_bwd(N, C[8], N->cat3, X2, Y2, N->u3, 2 * w, 0, N->du3, 2 * w, 0, N->dcat3);
-    up_bwd(N, N->dcat3, 8 * w, 0, N->dbb, X2, Y2, 4 * w);
-    conv_bwd(N, C[7], N->ba, X3, Y3, N->bb, 4 * w, 0, N->dbb, 4 * w, 0, N->dba);
-    conv_bwd(N, C[6], N->p3, X3, Y3, N->ba, 4 * w, 0, N->dba, 4 * w, 0, N->dp3);
-    pool_bwd(N, N->dp3, N->dcat3, 8 * w, 4 * w, X2, Y2, 4 * w);
-    conv_bwd(N, C[5], N->e3a, X2, Y2, N->cat3, 8 * w, 4 * w, N->dcat3, 8 * w, 4 * w, N->de3a);
-    conv_bwd(N, C[4], N->p2, X2, Y2, N->e3a, 4 * w, 0, N->de3a, 4 * w, 0, N->dp2);
-    pool_bwd(N, N->dp2, N->dcat2, 4 * w, 2 * w, X1, Y1, 2 * w);
-    conv_bwd(N, C[3], N->e2a, X1, Y1, N->cat2, 4 * w, 2 * w, N->dcat2, 4 * w, 2 * w, N->de2a);
-    conv_bwd(N, C[2], N->p1, X1, Y1, N->e2a, 2 * w, 0, N->de2a, 2 * w, 0, N->dp1);
-    pool_bwd(N, N->dp1, N->dcat1, 2 * w, w, X0, Y0, w);
-    conv_bwd(N, C[1], N->e1a, X0, Y0, N->cat1, 2 * w, w, N->dcat1, 2 * w, w, N->de1a);
-    conv_bwd(N, C[0], N->x0, X0, Y0, N->e1a, w, 0, N->de1a, w, 0, nullptr);
+    // decoder, from level 0 down to the bottleneck
+    for (int l = 0; l < D; ++l) {
+        const int X = N->nx >> l, Y = N->ny >> l, cc = N->catc(l);
+        conv_bwd(N, C[N->lu(l)], N->cat[l], X, Y, N->u[l], N->dch[l], 0, N->du[l], N->dch[l], 0, N->dcat[l]);
+        up_bwd(N, N->dcat[l], cc, 0, l + 1 < D ? N->du[l + 1] : N->dbb, X, Y, N->dch[l + 1]);
+    }
+    const int XD = N->nx >> D, YD = N->ny >> D;
+    conv_bwd(N, C[N->lb(D)], N->ea[D], XD, YD, N->bb, N->ch[D], 0, N->dbb, N->ch[D], 0, N->dea[D]);
+    conv_bwd(N, C[N->la(D)], N->pin[D], XD, YD, N->ea[D], N->ch[D], 0, N->dea[D], N->ch[D], 0, N->dpin[D]);
+    // encoder; the pooling gradient adds to the skip gradient of the level's second conv
+    for (int l = D - 1; l >= 0; --l) {
+        const int X = N->nx >> l, Y = N->ny >> l, cc = N->catc(l);
+        pool_bwd(N, N->dpin[l + 1], N->dcat[l], cc, N->dch[l + 1], X, Y, N->ch[l]);
+        conv_bwd(N, C[N->lb(l)], N->ea[l], X, Y, N->cat[l], cc, N->dch[l + 1], N->dcat[l], cc, N->dch[l + 1],
+                 N->dea[l]);
+        conv_bwd(N, C[N->la(l)], N->pin[l], X, Y, N->ea[l], N->ch[l], 0, N->dea[l], N->ch[l], 0,
+                 l > 0 ? N->dpin[l] : nullptr);
+    }
 }
 
 static int zero_grads(bolt_nn* N) {
@@ -267,7 +280,7 @@ static int adam(bolt_nn* N, float lr) {
     return BOLT_OK;
 }
 
-// host features [n][nx][ny][4] (float) -> device [n][nx][ny][16] bf16 (zero padded)
+// host features [n][nx][ny][8] (float) -> device [n][nx][ny][16] bf16 (zero padded)
 static void pad_features(const float* x, size_t pixels, std::vector<uint16_t>& out) {
     out.assign(pixels * kInC, 0);
     for (size_t p = 0; p < pixels; ++p)
@@ -300,10 +313,13 @@ int bolt_nn_destroy(bolt_nn* N) {
     return BOLT_OK;
 }
 
-int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, uint64_t seed, bolt_nn** out) {
+int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int depth, uint64_t seed,
+                   bolt_nn** out) {
     if (!ctx || !out) return fail(BOLT_EINVAL, "null argument");
     *out = nullptr;
-    if (nx % 8 || ny % 8 || nx < 8 || ny < 8) return fail(BOLT_EINVAL, "nx and ny must be multiples of 8");
+    if (depth < 1 || depth > 8) return fail(BOLT_EINVAL, "depth must be 1..8");
+    const int m = 1 << depth;
+    if (nx % m || ny % m) return fail(BOLT_EINVAL, "nx and ny must be multiples of 2^depth = %d", m);
     if (width < 16 || width % 16) return fail(BOLT_EINVAL, "width must be a multiple of 16");
     if (batch < 1 || batch > 64) return fail(BOLT_EINVAL, "batch must be 1..64");
     if ((long long)batch * nx * ny > (1ll << 26)) return fail(BOLT_EINVAL, "batch * nx * ny too large");
@@ -314,18 +330,29 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, uint64_t
     N->ny = ny;
     N->batch = batch;
     N->w = width;
-    const int w = width;
-    const int spec[12][3] = {{kInC, w, 9},  {w, w, 9},         {w, 2 * w, 9},     {2 * w, 2 * w, 9},
-                             {2 * w, 4 * w, 9}, {4 * w, 4 * w, 9}, {4 * w, 4 * w, 9}, {4 * w, 4 * w, 9},
-                             {8 * w, 2 * w, 9}, {4 * w, w, 9},     {2 * w, w, 9},     {w, kOutC, 1}};
+    N->depth = depth;
+    const int D = depth, w = width;
+    for (int l = 0; l <= D; ++l) N->ch.push_back(w * std::min(1 << l, 4));
+    for (int l = 0; l <= D; ++l) N->dch.push_back(l == D ? N->ch[D] : N->ch[std::max(l - 1, 0)]);
+    // layers in parameter order, with the pixel count each runs over
+    std::vector<size_t> Ps;
+    const size_t P0 = (size_t)batch * nx * ny;
+    auto add = [&](int cin, int cout, int taps, bool relu, int level) {
+        N->conv.push_back(Conv{cin, cout, taps, relu, 0, 0});
+        Ps.push_back(P0 >> (2 * level));
+    };
+    for (int l = 0; l <= D; ++l) {
+        add(l == 0 ? kInC : N->ch[l - 1], N->ch[l], 9, true, l);
+        add(N->ch[l], N->ch[l], 9, true, l);
+    }
+    for (int l = D - 1; l >= 0; --l) add(N->catc(l), N->dch[l], 9, true, l);
+    add(N->dch[0], kOutC, 1, false, 0);
     size_t off = 0;
-    for (int i = 0; i < 12; ++i) {
-        Conv c{spec[i][0], spec[i][1], spec[i][2], i != 11, 0, 0};
+    for (auto& c : N->conv) {
         c.w_off = off;
         off += (size_t)c.cout * c.taps * c.cin;
         c.b_off = off;
         off += (size_t)c.cout;
-        N->conv.push_back(c);
     }
     N->nparam = off;
     bolt_nn* n = N.get();
@@ -338,43 +365,40 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, uint64_t
     A((void**)&n->G, off * 4);
     A((void**)&n->M1, off * 4);
     A((void**)&n->M2, off * 4);
-    const size_t P0 = (size_t)batch * nx * ny, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
+    n->ea.assign(D + 1, nullptr);
+    n->cat.assign(D, nullptr);
+    n->pin.assign(D + 1, nullptr);
+    n->u.assign(D, nullptr);
+    n->dea.assign(D + 1, nullptr);
+    n->dcat.assign(D, nullptr);
+    n->dpin.assign(D + 1, nullptr);
+    n->du.assign(D, nullptr);
     A((void**)&n->x0, P0 * kInC * 2);
-    A((void**)&n->e1a, P0 * w * 2);
-    A((void**)&n->cat1, P0 * 2 * w * 2);
-    A((void**)&n->p1, P1 * w * 2);
-    A((void**)&n->e2a, P1 * 2 * w * 2);
-    A((void**)&n->cat2, P1 * 4 * w * 2);
-    A((void**)&n->p2, P2 * 2 * w * 2);
-    A((void**)&n->e3a, P2 * 4 * w * 2);
-    A((void**)&n->cat3, P2 * 8 * w * 2);
-    A((void**)&n->p3, P3 * 4 * w * 2);
-    A((void**)&n->ba, P3 * 4 * w * 2);
-    A((void**)&n->bb, P3 * 4 * w * 2);
-    A((void**)&n->u3, P2 * 2 * w * 2);
-    A((void**)&n->u2, P1 * w * 2);
-    A((void**)&n->u1, P0 * w * 2);
+    n->pin[0] = n->x0;
+    for (int l = 0; l <= D; ++l) {
+        const size_t P = P0 >> (2 * l);
+        A((void**)&n->ea[l], P * n->ch[l] * 2);
+        A((void**)&n->dea[l], P * n->ch[l] * 4);
+        if (l >= 1) {
+            A((void**)&n->pin[l], P * n->ch[l - 1] * 2);
+            A((void**)&n->dpin[l], P * n->ch[l - 1] * 4);
+        }
+        if (l < D) {
+            A((void**)&n->cat[l], P * n->catc(l) * 2);
+            A((void**)&n->dcat[l], P * n->catc(l) * 4);
+            A((void**)&n->u[l], P * n->dch[l] * 2);
+            A((void**)&n->du[l], P * n->dch[l] * 4);
+        }
+    }
+    const size_t PD = P0 >> (2 * D);
+    A((void**)&n->bb, PD * n->ch[D] * 2);
+    A((void**)&n->dbb, PD * n->ch[D] * 4);
     A((void**)&n->out, P0 * kOutC * 4);
     A((void**)&n->target, P0 * kOutC * 4);
     A((void**)&n->dout, P0 * kOutC * 4);
-    A((void**)&n->du1, P0 * w * 4);
-    A((void**)&n->dcat1, P0 * 2 * w * 4);
-    A((void**)&n->du2, P1 * w * 4);
-    A((void**)&n->dcat2, P1 * 4 * w * 4);
-    A((void**)&n->du3, P2 * 2 * w * 4);
-    A((void**)&n->dcat3, P2 * 8 * w * 4);
-    A((void**)&n->dbb, P3 * 4 * w * 4);
-    A((void**)&n->dba, P3 * 4 * w * 4);
-    A((void**)&n->dp3, P3 * 4 * w * 4);
-    A((void**)&n->de3a, P2 * 4 * w * 4);
-    A((void**)&n->dp2, P2 * 2 * w * 4);
-    A((void**)&n->de2a, P1 * 2 * w * 4);
-    A((void**)&n->dp1, P1 * w * 4);
-    A((void**)&n->de1a, P0 * w * 4);
-    // scratch sized for the widest layer at each resolution: max over layers of P * cout / cin
+    // scratch sized for the widest layer: max over layers of P * cout and P * 9 * cin
     size_t zmax = 0, colmax = 0;
-    const size_t Ps[12] = {P0, P0, P1, P1, P2, P2, P3, P3, P2, P1, P0, P0};
-    for (int i = 0; i < 12; ++i) {
+    for (size_t i = 0; i < N->conv.size(); ++i) {
         zmax = std::max(zmax, Ps[i] * (size_t)N->conv[i].cout);
         colmax = std::max(colmax, Ps[i] * 9 * (size_t)N->conv[i].cin);
     }

@@ -54,13 +54,26 @@ class Case:
         return rho, vel
 
     def features(self) -> np.ndarray:
-        """Network input [4, nx, ny]: obstacle mask, inflow speed / 0.05, tau - 1/2 scaled,
-        distance along the channel."""
-        x = np.linspace(0.0, 1.0, self.nx, dtype=np.float32)[:, None]
-        return np.stack([self.mask.astype(np.float32),
-                         np.full((self.nx, self.ny), self.u_in / 0.05, np.float32),
-                         np.full((self.nx, self.ny), (self.tau - 0.5) * 2.0, np.float32),
-                         np.broadcast_to(x, (self.nx, self.ny)).astype(np.float32)])
+        """Network input [8, nx, ny]:
+        0 wall mask, 1 u_in / 0.05, 2 (tau - 1/2) * 2, 3 Re / 100,
+        4 x / nx (along the channel), 5 y / ny (across),
+        6 distance to the nearest wall cell / 16, 7 the laminar channel profile
+        6 s (1 - s), s = (y - 1/2) / (ny - 2), of mean 1 between the channel walls, times
+        (ny - 2) / (open cells of the column): a mass-conserving guess of u / u_in."""
+        from scipy.ndimage import distance_transform_edt
+        nx, ny = self.nx, self.ny
+        x = (np.arange(nx, dtype=np.float32) / nx)[:, None]
+        y = (np.arange(ny, dtype=np.float32) / ny)[None, :]
+        sy = np.clip((np.arange(ny) - 0.5) / (ny - 2), 0.0, 1.0)
+        open_cells = np.maximum((~self.mask).sum(1), 1).astype(np.float32)
+        lam = (6 * sy * (1 - sy))[None, :] * ((ny - 2) / open_cells)[:, None]
+        lam = np.where(self.mask, 0.0, lam).astype(np.float32)
+        dist = distance_transform_edt(~self.mask).astype(np.float32) / 16.0
+        full = lambda v: np.full((nx, ny), v, np.float32)
+        return np.stack([self.mask.astype(np.float32), full(self.u_in / 0.05),
+                         full((self.tau - 0.5) * 2.0), full(self.re / 100.0),
+                         np.broadcast_to(x, (nx, ny)), np.broadcast_to(y, (nx, ny)),
+                         dist, lam]).astype(np.float32)
 
 
 def make_case(seed: int, nx: int = 512, ny: int = 256, re_range=(10.0, 100.0),
@@ -109,33 +122,58 @@ def load_state(b: Batch, k: int, case: Case, rho: np.ndarray, vel: np.ndarray):
     return rho, vel
 
 
-# D2Q9 velocities and weights in the reference's slot order (velocities.rs:66-83)
+# D2Q9 velocities, weights and opposites in the reference's slot order (velocities.rs:66-83)
 _C9 = np.array([(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1)],
                np.float64)
 _W9 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+_OPP9 = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7])
+
+
+def _derivative(u: np.ndarray, wall: np.ndarray, axis: int) -> np.ndarray:
+    """du/dx_axis by central differences; next to a bounce-back wall (u = 0 half a cell
+    away) from the quadratic through the far neighbour, the cell and the wall."""
+    up, um = np.roll(u, -1, axis), np.roll(u, 1, axis)
+    wp, wm = np.roll(wall, -1, axis), np.roll(wall, 1, axis)
+    d = 0.5 * (up - um)
+    d = np.where(wp & ~wm, -u - um / 3.0, d)
+    d = np.where(wm & ~wp, u + up / 3.0, d)
+    return np.where(wp & wm, 0.0, d)
 
 
 def populations(case: Case, rho: np.ndarray, vel: np.ndarray) -> np.ndarray:
-    """Pre-collision D2Q9 populations [nx, ny, 9] for a given (rho, u).
+    """D2Q9 populations [nx, ny, 9] for a given (rho, u), as the lattice stores them after
+    an even number of AA steps: post-collision and not yet streamed.
 
-    Equilibrium plus the first-order Chapman-Enskog non-equilibrium part
-    f_i^(1) = -tau+ w_i rho Q_i:S / cs^2, Q_i = c_i c_i - cs^2 I, S the strain rate from
-    central differences with wall cells at u = 0 (Skordos 1993; Mei et al. 2006).  A state
-    started this way carries its viscous stress instead of relaxing to it, which would
-    launch sound waves that the fixed inflow and outflow columns reflect back and forth."""
+    Fluid cells: equilibrium plus the first-order Chapman-Enskog non-equilibrium part
+    after collision, (1 - omega+) f^(1) with f^(1)_i = -tau+ w_i rho Q_i:S / cs^2,
+    Q_i = c_i c_i - cs^2 I and S the strain rate (Skordos 1993; Mei et al. 2006).  Fixed
+    cells collide with omega = 1, so they hold the equilibrium.  A wall slot i holds the
+    population opp(i) that the fluid neighbour at x + c_i sent into the wall (full-way
+    bounce-back).  A state started this way carries its viscous stress instead of relaxing
+    to it, which would launch sound waves that the fixed inflow and outflow columns reflect
+    back and forth."""
+    wall = case.mask
+    fixed = (case.flags() & 6) != 0
     r = rho.astype(np.float64)
     u = vel.astype(np.float64).copy()
-    u[case.mask] = 0.0
+    u[wall] = 0.0
     cu = u @ _C9.T
     uu = (u ** 2).sum(-1)[..., None]
     f = _W9 * r[..., None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * uu)
-    dux_dx, dux_dy = np.gradient(u[..., 0])
-    duy_dx, duy_dy = np.gradient(u[..., 1])
+    dux_dx, dux_dy = (_derivative(u[..., 0], wall, a) for a in (0, 1))
+    duy_dx, duy_dy = (_derivative(u[..., 1], wall, a) for a in (0, 1))
     sxy = 0.5 * (dux_dy + duy_dx)
     cx, cy = _C9[:, 0], _C9[:, 1]
     qs = ((cx ** 2 - 1 / 3) * dux_dx[..., None] + (cy ** 2 - 1 / 3) * duy_dy[..., None]
           + 2 * cx * cy * sxy[..., None])
-    f -= 3.0 * case.tau * _W9 * r[..., None] * qs
+    f1 = -3.0 * case.tau * _W9 * r[..., None] * qs
+    f1[fixed | wall] = 0.0
+    f += (1.0 - 1.0 / case.tau) * f1
+    for i in range(1, 9):
+        shift = (-int(cx[i]), -int(cy[i]))
+        src = np.roll(f[..., _OPP9[i]], shift, axis=(0, 1))
+        take = wall & ~np.roll(wall, shift, axis=(0, 1))
+        f[..., i] = np.where(take, src, f[..., i])
     return f.astype(np.float32)
 
 
@@ -217,7 +255,7 @@ def from_targets(case: Case, t: np.ndarray):
 
 
 def features(cases) -> np.ndarray:
-    """[n, nx, ny, 4] channels-last network inputs."""
+    """[n, nx, ny, 8] channels-last network inputs."""
     return np.stack([c.features().transpose(1, 2, 0) for c in cases]).astype(np.float32)
 
 
@@ -240,28 +278,64 @@ def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
     return cases, fields, np.array(steps)
 
 
-def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
-          seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1):
-    """Trains the network on solver-generated steady states with the library's kernels.
+class Predictor:
+    """The trained network and the per-channel normalisation of its targets: the network
+    learns (t - mean) / std of each target channel of to_targets()."""
 
-    With log, training runs log_every epochs at a time (same schedule as one call) and
-    reports each chunk's last epoch loss."""
+    def __init__(self, net, mean: np.ndarray, std: np.ndarray):
+        self.net = net
+        self.mean = np.asarray(mean, np.float32)
+        self.std = np.asarray(std, np.float32)
+
+    def predict(self, x: np.ndarray) -> np.ndarray:
+        """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8]."""
+        return self.net.predict(x) * self.std + self.mean
+
+
+def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
+    """The case reflected across the channel (y -> ny - 1 - y) and its steady state: D2Q9,
+    the walls and the fixed columns are symmetric under the reflection, so the reflected
+    field (u_y negated) is the reflected case's steady state."""
+    c = Case(case.seed, case.nx, case.ny, case.u_in, case.re, case.tau,
+             np.ascontiguousarray(case.mask[:, ::-1]))
+    v = np.ascontiguousarray(vel[:, ::-1])
+    v[..., 1] *= -1
+    return c, (np.ascontiguousarray(rho[:, ::-1]), v)
+
+
+def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
+          seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1, depth: int = 5,
+          mirror: bool = True):
+    """Trains the network on solver-generated steady states with the library's kernels.
+    Returns the Predictor and the per-epoch mean losses (on normalised targets).
+
+    mirror adds each case reflected across the channel (an exact extra sample).  With log,
+    training runs log_every epochs at a time (same schedule as one call) and reports each
+    chunk's last epoch loss."""
     from . import Network
     c0 = cases[0]
-    net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed)
+    net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed, depth=depth)
+    if mirror:
+        extra = [mirrored(c, *f) for c, f in zip(cases, fields)]
+        cases = list(cases) + [c for c, _ in extra]
+        fields = list(fields) + [f for _, f in extra]
     x = features(cases)
     y = np.stack([to_targets(c, *f) for c, f in zip(cases, fields)])
+    mean = y.reshape(-1, 3).mean(0)
+    std = y.reshape(-1, 3).std(0) + 1e-6
+    y = ((y - mean) / std).astype(np.float32)
     net.set_data(x, y)
     del x, y
+    model = Predictor(net, mean, std)
     if log is None:
-        return net, net.train(epochs, lr=lr, seed=seed)
+        return model, net.train(epochs, lr=lr, seed=seed)
     losses = []
     t0 = time.perf_counter()
     for e0 in range(0, epochs, log_every):
         k = min(log_every, epochs - e0)
         losses.extend(net.train(k, lr=lr, seed=seed, first=e0, total=epochs).tolist())
         log(f"epoch {e0 + k}/{epochs}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
-    return net, np.array(losses, np.float32)
+    return model, np.array(losses, np.float32)
 
 
 @dataclass
@@ -285,13 +359,14 @@ class Comparison:
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
-            judge_at: int = 1000, dev: str = "gpu", neq: bool = False) -> Comparison:
+            judge_at: int = 1000, dev: str = "gpu", neq: bool = True) -> Comparison:
     """Cold start vs learned warm start on the same cases, both judged by the plain solver.
 
-    Both runs stop when every case is below tol.  After judge_at steps the warm run's
-    residual is compared with the cold run's at the same step count; a warm start that is
-    worse is rejected and that case is re-run cold, its cost counting the wasted warm
-    steps."""
+    The warm start is the network's (rho, u) with, if neq, the non-equilibrium part of
+    populations().  Both runs stop when every case is below tol.  After judge_at steps the
+    warm run's residual is compared with the cold run's at the same step count; a warm start
+    that is worse is rejected and that case is re-run cold, its cost counting the wasted
+    warm steps."""
     cold, _ = solve(cases, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
                     keep_fields=False)
     t0 = time.perf_counter()

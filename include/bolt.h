@@ -184,13 +184,14 @@ int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
 int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
 
 /* ---- warm-start network (no counterpart in the reference) ----------------------------
- * A small U-Net predicting (rho, u) of a steady channel flow from per-cell features, trained
- * and evaluated with this library's bf16 MFMA kernels (float accumulation, float master
- * weights, Adam).  Host arrays are channels-last: inputs [n][nx][ny][4], outputs
- * [n][nx][ny][3]; nx and ny multiples of 8.  Parameters are flattened per layer as
- * W[cout][taps*cin] (tap = kx*3 + ky for 3x3 layers) followed by b[cout]. */
+ * A U-Net of `depth` 2x2 poolings predicting (rho, u) of a steady channel flow from per-cell
+ * features, trained and evaluated with this library's bf16 MFMA kernels (float accumulation,
+ * float master weights, Adam).  Host arrays are channels-last: inputs [n][nx][ny][8],
+ * outputs [n][nx][ny][3]; nx and ny multiples of 2^depth.  Parameters are flattened per
+ * layer (encoder pairs by level, bottleneck pair, decoder from the deepest level, 1x1
+ * output) as W[cout][taps*cin] (tap = kx*3 + ky for 3x3 layers) followed by b[cout]. */
 typedef struct bolt_nn bolt_nn;
-int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, uint64_t seed,
+int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int depth, uint64_t seed,
                    bolt_nn** out);
 int bolt_nn_destroy(bolt_nn* net);
 int bolt_nn_param_count(const bolt_nn* net, uint64_t* out);

@@ -42,7 +42,21 @@ def _bf16(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
-def torch_forward(params, x, w, emulate=True):
+def layer_spec(w, depth):
+    """(cin, cout, taps) in parameter order: encoder pairs by level, bottleneck pair,
+    decoder from the deepest level up, 1x1 output (channels w * min(2^l, 4))."""
+    ch = [w * min(2 ** l, 4) for l in range(depth + 1)]
+    dch = [ch[max(l - 1, 0)] for l in range(depth)] + [ch[depth]]
+    spec = []
+    for l in range(depth + 1):
+        spec += [(16 if l == 0 else ch[l - 1], ch[l], 9), (ch[l], ch[l], 9)]
+    for l in range(depth - 1, -1, -1):
+        spec.append((dch[l + 1] + ch[l], dch[l], 9))
+    spec.append((dch[0], 3, 1))
+    return spec
+
+
+def torch_forward(params, x, w, depth=3, emulate=True):
     """The network of bolt_nn.hip in torch (NCHW, float32).  With emulate, values are rounded
     to bf16 where the GPU kernels store bf16: weights and the input, every layer's stored
     activation, and (in the backward pass) every pre-activation gradient."""
@@ -60,9 +74,7 @@ def torch_forward(params, x, w, emulate=True):
 
     R = _bf16 if emulate else (lambda t: t)
     G = GradRound.apply if emulate else (lambda t: t)
-    spec = [(16, w, 9), (w, w, 9), (w, 2 * w, 9), (2 * w, 2 * w, 9), (2 * w, 4 * w, 9),
-            (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (4 * w, 4 * w, 9), (8 * w, 2 * w, 9),
-            (4 * w, w, 9), (2 * w, w, 9), (w, 3, 1)]
+    spec = layer_spec(w, depth)
     Ws, bs, off = [], [], 0
     for cin, cout, taps in spec:
         Wf = params[off:off + cout * taps * cin]
@@ -76,38 +88,46 @@ def torch_forward(params, x, w, emulate=True):
         bs.append(torch.tensor(np.ascontiguousarray(b), requires_grad=True))
     assert off == params.size
     xt = np.zeros((x.shape[0], 16) + x.shape[1:3], np.float32)
-    xt[:, :4] = x.transpose(0, 3, 1, 2)
+    xt[:, :8] = x.transpose(0, 3, 1, 2)
     X = R(torch.tensor(xt))
+    last = len(spec) - 1
 
     def c(i, t):
-        y = G(F.conv2d(t, R(Ws[i]), bs[i], padding=1 if i < 11 else 0))
-        return R(F.relu(y)) if i < 11 else y
+        y = G(F.conv2d(t, R(Ws[i]), bs[i], padding=1 if i < last else 0))
+        return R(F.relu(y)) if i < last else y
 
     up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")
     pool = lambda t: R(F.avg_pool2d(t, 2))
-    e1b = c(1, c(0, X))
-    e2b = c(3, c(2, pool(e1b)))
-    e3b = c(5, c(4, pool(e2b)))
-    bb = c(7, c(6, pool(e3b)))
-    u3 = c(8, torch.cat([up(bb), e3b], 1))
-    u2 = c(9, torch.cat([up(u3), e2b], 1))
-    u1 = c(10, torch.cat([up(u2), e1b], 1))
-    out = c(11, u1)
+    skips, t = [], X
+    for l in range(depth):
+        t = c(2 * l + 1, c(2 * l, t))
+        skips.append(t)
+        t = pool(t)
+    t = c(2 * depth + 1, c(2 * depth, t))
+    for k, l in enumerate(range(depth - 1, -1, -1)):
+        t = c(2 * depth + 2 + k, torch.cat([up(t), skips[l]], 1))
+    out = c(last, t)
     return out, Ws, bs, spec
 
 
-def test_network_loss_and_gradients_match_torch():
+# (depth, nx, ny, min cosine, max relative error) per layer.  The deepest levels of the
+# depth-5 net hold 4 x 2 pixels, too few to average out where bf16 rounding of a near tie
+# goes the other way than in the reference; a wrong gradient scores cos < 0.9.
+@pytest.mark.parametrize("depth,nx,ny,cos_min,rel_max", [(3, 32, 16, 0.999, 0.05),
+                                                         (5, 128, 64, 0.995, 0.1)])
+def test_network_loss_and_gradients_match_torch(depth, nx, ny, cos_min, rel_max):
     import torch
     import boltzmann_amd
-    nx, ny, B, w = 32, 16, 2, 16
-    net = boltzmann_amd.Network("gpu", nx, ny, B, width=w, seed=3)
+    B, w = 2, 16
+    net = boltzmann_amd.Network("gpu", nx, ny, B, width=w, seed=3, depth=depth)
     params = net.get_params()
+    assert params.size == sum(co * t * ci + co for ci, co, t in layer_spec(w, depth))
     rng = np.random.default_rng(0)
-    x = rng.standard_normal((B, nx, ny, 4)).astype(np.float32)
+    x = rng.standard_normal((B, nx, ny, 8)).astype(np.float32)
     y = rng.standard_normal((B, nx, ny, 3)).astype(np.float32)
     loss, grads = net.loss_grad(x, y)
 
-    out, Ws, bs, spec = torch_forward(params, x, w)
+    out, Ws, bs, spec = torch_forward(params, x, w, depth)
     tl = torch.mean((out - torch.tensor(y.transpose(0, 3, 1, 2))) ** 2)
     tl.backward()
     assert abs(loss - tl.item()) / tl.item() < 2e-3, (loss, tl.item())
@@ -128,7 +148,7 @@ def test_network_loss_and_gradients_match_torch():
             report.append((i, round(cos, 5), round(rel, 4)))
     print("per-layer (cos, rel) of W then b:", report, flush=True)
     for i, cos, rel in report:
-        assert cos > 0.999 and rel < 0.05, report
+        assert cos > cos_min and rel < rel_max, report
 
 
 def test_training_reduces_loss():
@@ -137,7 +157,7 @@ def test_training_reduces_loss():
     net = boltzmann_amd.Network("gpu", nx, ny, B, width=16, seed=1)
     rng = np.random.default_rng(1)
     n = 16
-    x = rng.standard_normal((n, nx, ny, 4)).astype(np.float32)
+    x = rng.standard_normal((n, nx, ny, 8)).astype(np.float32)
     # a smooth target that depends on the input: a learnable map
     y = np.stack([np.tanh(x[..., 0]), 0.5 * x[..., 1], x[..., 0] * x[..., 2] * 0.1], -1).astype(np.float32)
     net.set_data(x, y)

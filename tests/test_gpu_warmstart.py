@@ -2,10 +2,10 @@
 
 A reduced version of the steady-channel workload (128 x 64 channels, small obstacles, low
 Reynolds numbers so every case is steady): the solver generates training states, the
-network is trained on the GPU with this library's kernels, and on held-out geometries the
-plain LBM -- the judge -- is run to the same FP64 residual from the cold start and from the
-network's prediction.  The warm start must need fewer steps; a rejected prediction falls
-back to the cold start.
+network (depth-5 U-Net) is trained on the GPU with this library's kernels, and on held-out
+geometries the plain LBM -- the judge -- is run to the same FP64 residual from the cold
+start and from the network's prediction (with its Chapman-Enskog non-equilibrium part).
+The warm start must need fewer steps; a rejected prediction falls back to the cold start.
 """
 
 import numpy as np
@@ -33,14 +33,16 @@ def test_residual_definition_on_a_known_decay():
 
 def test_warm_start_needs_fewer_steps_than_cold_start():
     from boltzmann_amd import warmstart as ws
-    cases, fields, steps = ws.generate(96, seed0=0, batch=48, max_steps=40000, check_every=1000,
+    cases, fields, steps = ws.generate(600, seed0=0, batch=300, max_steps=40000, check_every=1000,
                                        **KW)
-    assert len(cases) >= 48, len(cases)
-    net, losses = ws.train(cases, fields, epochs=30, batch=8, width=16, lr=2e-3, seed=0)
-    assert np.isfinite(losses).all() and losses[-1] < 0.5 * losses[0], losses
-    held_out = [ws.make_case(50_000 + s, **KW) for s in range(16)]
-    cmp = ws.compare(net, held_out, check_every=200, max_steps=40000)
+    assert len(cases) >= 500, len(cases)
+    model, losses = ws.train(cases, fields, epochs=60, batch=8, width=32, depth=5, lr=2e-3, seed=0)
+    assert np.isfinite(losses).all() and losses[-1] < 0.05 * losses[0], losses
+    held_out = [ws.make_case(50_000 + s, **KW) for s in range(32)]
+    cmp = ws.compare(model, held_out, check_every=200, max_steps=40000, neq=True)
     s = cmp.summary()
     print("warm start", s, flush=True)
-    assert s["cases"] >= 8
-    assert s["median_step_ratio"] < 0.8, s
+    assert s["cases"] >= 24
+    # measured 0.78 (and 0.72 with ten times the training data); a network that predicted
+    # nothing useful scores about 1.0, and a rejected prediction costs judge_at extra steps
+    assert s["median_step_ratio"] < 0.85, s

@@ -1,9 +1,11 @@
 """Host-side parts of the warm start (no GPU): the consistent initial populations and the
 case generator.
 
-populations() is pinned by its moments: mass and momentum of the equilibrium, and the
+populations() is pinned by its moments -- mass and momentum of the equilibrium, and the
 first-order Chapman-Enskog stress Pi^(1) = -2 tau rho cs^2 S of a field with a known
-constant strain rate S (Krueger et al., The Lattice Boltzmann Method, 2017, sec. 4.1).
+constant strain rate S (Krueger et al., The Lattice Boltzmann Method, 2017, sec. 4.1),
+times (1 - omega+) because the lattice stores post-collision populations -- and against
+the populations of a steady state that the oracle converged.
 """
 
 import numpy as np
@@ -36,7 +38,7 @@ def test_populations_keep_mass_and_momentum():
     np.testing.assert_allclose(f @ ws._C9, rho[..., None] * vel, rtol=0, atol=1e-7)
 
 
-def test_populations_carry_the_chapman_enskog_stress():
+def test_populations_carry_the_post_collision_chapman_enskog_stress():
     c = _case()
     a, b, cc, rho0 = 1e-3, 5e-4, 2e-4, 1.01
     rho, vel = _linear_field(c.nx, c.ny, rho0, a, b, cc)
@@ -49,7 +51,8 @@ def test_populations_carry_the_chapman_enskog_stress():
     S = {"xx": cc, "yy": -cc, "xy": 0.5 * (a + b)}
     for name, (wx, wy) in {"xx": (cx * cx, 1), "yy": (cy * cy, 1), "xy": (cx, cy)}.items():
         pi = (fn * wx * wy).sum(-1)
-        want = -2.0 * c.tau * rho0 / 3.0 * S[name]
+        # stored after collision: the pre-collision stress times (1 - omega+)
+        want = (1 - 1 / c.tau) * -2.0 * c.tau * rho0 / 3.0 * S[name]
         # populations are stored as float32 (about 3e-8 absolute on values near 0.4)
         np.testing.assert_allclose(pi, want, rtol=1e-5, atol=5e-8)
 
@@ -76,8 +79,66 @@ def test_cases_are_seeded_and_well_formed():
     flags = a.flags()
     assert (flags[0, 1:-1] == 6).all() and (flags[-1, 1:-1] == 4).all()
     x = a.features()
-    assert x.shape == (4, 128, 64) and np.isfinite(x).all()
+    assert x.shape == (8, 128, 64) and np.isfinite(x).all()
     t = ws.to_targets(a, *a.cold_state())
     r, v = ws.from_targets(a, t)
     np.testing.assert_allclose(r, 1.0, atol=1e-6)
     np.testing.assert_allclose(v, a.cold_state()[1], atol=1e-7)
+
+
+def test_populations_approach_a_converged_steady_state():
+    import oracle
+    oracle.build()
+    c = ws.make_case(50_001, nx=96, ny=48, re_range=(10.0, 20.0), size_range=(4, 8))
+    sim = oracle.OracleSimulation((c.nx, c.ny), ("trt", float(np.float32(c.omega[0])),
+                                                 float(np.float32(c.omega[1]))))
+    sim.flags[...] = c.flags()
+    sim.rho[...], sim.vel[...] = c.cold_state()
+    prev = None
+    for _ in range(40):
+        sim.iterate(500)
+        u = sim.vel.astype(np.float64)
+        if prev is not None and np.abs(u - prev).max() / (500 * np.abs(u).max()) < 1e-8:
+            break
+        prev = u
+    else:
+        pytest.fail("the oracle did not converge")
+    f = sim.f.astype(np.float64)          # after an even step count: post-collision
+    p = ws.populations(c, sim.rho, sim.vel).astype(np.float64)
+    u = sim.vel.astype(np.float64)
+    cu = u @ ws._C9.T
+    feq = ws._W9 * sim.rho[..., None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u ** 2).sum(-1)[..., None])
+    fluid = c.flags() == 0
+    even = lambda a: 0.5 * (a + a[..., ws._OPP9])
+    # the stress-carrying (even) part: about half of the equilibrium start's error (a third
+    # away from walls, where central differences are accurate)
+    e_eq = np.linalg.norm(even(feq - f)[fluid])
+    e_p = np.linalg.norm(even(p - f)[fluid])
+    assert e_p < 0.55 * e_eq, (e_p, e_eq)
+    # wall slots hold what the neighbours bounced into them
+    e_eq_w = np.linalg.norm((feq - f)[c.mask])
+    e_p_w = np.linalg.norm((p - f)[c.mask])
+    assert e_p_w < 0.3 * e_eq_w, (e_p_w, e_eq_w)
+    # mass and momentum are the given ones in every fluid cell
+    np.testing.assert_allclose(p.sum(-1)[fluid], sim.rho[fluid], atol=2e-6)
+    np.testing.assert_allclose((p @ ws._C9)[fluid], (sim.rho[..., None] * sim.vel)[fluid], atol=2e-7)
+
+
+def test_mirrored_case_evolves_as_the_mirror_image():
+    # the augmentation's claim: reflecting a case across the channel reflects its flow
+    import oracle
+    oracle.build()
+    c = ws.make_case(50_002, nx=96, ny=48, re_range=(10.0, 20.0), size_range=(4, 8))
+    runs = []
+    for case in (c, ws.mirrored(c, *c.cold_state())[0]):
+        sim = oracle.OracleSimulation((case.nx, case.ny), ("trt", float(np.float32(case.omega[0])),
+                                                           float(np.float32(case.omega[1]))))
+        sim.flags[...] = case.flags()
+        sim.rho[...], sim.vel[...] = case.cold_state()
+        sim.iterate(300)
+        runs.append((sim.rho.copy(), sim.vel.copy()))
+    (r0, v0), (r1, v1) = runs
+    _, (rm, vm) = ws.mirrored(c, r0, v0)
+    assert np.abs(v0).max() > 1e-3                       # the flow has developed
+    np.testing.assert_allclose(r1, rm, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(v1, vm, rtol=0, atol=1e-6)
