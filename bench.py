@@ -186,13 +186,15 @@ def cylinder(steps_marks=(1000, 5000, 20000)):
 
 
 def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
-                      width: int = 32, max_steps: int = 60000):
+                      width: int = 32, depth: int = 6, max_steps: int = 60000):
     """Learned warm start against the plain solver's time to steady state.
 
     D2Q9 TRT channels of 512 x 256 cells with 1-8 seeded obstacles (radius / half-size
     8-32), u_in in [0.02, 0.05], Re in [10, 100].  n_train cases (seeds 0..) are solved to
     the FP64 residual 1e-6 in batches; those that converge within max_steps train the
-    network; n_test held-out cases (seeds 10^6..) run cold and warm to the same residual."""
+    network (with their mirror images across the channel); n_test held-out cases
+    (seeds 10^6..) run cold and warm (prediction plus its Chapman-Enskog non-equilibrium
+    part) to the same residual."""
     from boltzmann_amd import warmstart as ws
 
     def log(msg):
@@ -203,8 +205,8 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
                                        check_every=2000, log=log)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
-    net, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=width, lr=1e-3,
-                           seed=0, log=log)
+    net, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=width, depth=depth,
+                           lr=2e-3, seed=0, log=log)
     t_train = time.perf_counter() - t0
     del fields
     held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
@@ -214,7 +216,8 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
     return {"train_cases": len(cases), "train_requested": n_train,
             "train_steps_median": int(np.median(steps)) if len(steps) else None,
             "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
-            "epochs": epochs, "width": width, "params": int(net.net.param_count),
+            "epochs": epochs, "width": width, "depth": depth,
+            "params": int(net.net.param_count),
             "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
             "held_out": s}
 

@@ -291,6 +291,19 @@ class Predictor:
         """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8]."""
         return self.net.predict(x) * self.std + self.mean
 
+    def predict_cases(self, cases, mirror: bool = True) -> np.ndarray:
+        """Targets for cases; with mirror, the mean of the prediction and the reflected
+        prediction of the reflected case (exact symmetry, so only the network's error
+        differs between the two)."""
+        t = self.predict(features(cases))
+        if not mirror:
+            return t
+        mc = [Case(c.seed, c.nx, c.ny, c.u_in, c.re, c.tau, np.ascontiguousarray(c.mask[:, ::-1]))
+              for c in cases]
+        tm = self.predict(features(mc))[:, :, ::-1].copy()
+        tm[..., 1] *= -1
+        return 0.5 * (t + tm)
+
 
 def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
     """The case reflected across the channel (y -> ny - 1 - y) and its steady state: D2Q9,
@@ -305,11 +318,13 @@ def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
 
 def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
           seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1, depth: int = 5,
-          mirror: bool = True):
+          mirror: bool = True, rho_weight: float = 1 / 9):
     """Trains the network on solver-generated steady states with the library's kernels.
     Returns the Predictor and the per-epoch mean losses (on normalised targets).
 
-    mirror adds each case reflected across the channel (an exact extra sample).  With log,
+    mirror adds each case reflected across the channel (an exact extra sample).  rho_weight
+    scales the density channel's share of the loss (its errors cost the warm start far
+    fewer steps than velocity errors).  With log,
     training runs log_every epochs at a time (same schedule as one call) and reports each
     chunk's last epoch loss."""
     from . import Network
@@ -323,6 +338,7 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
     y = np.stack([to_targets(c, *f) for c, f in zip(cases, fields)])
     mean = y.reshape(-1, 3).mean(0)
     std = y.reshape(-1, 3).std(0) + 1e-6
+    std[2] /= np.sqrt(rho_weight)
     y = ((y - mean) / std).astype(np.float32)
     net.set_data(x, y)
     del x, y
@@ -359,10 +375,12 @@ class Comparison:
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
-            judge_at: int = 1000, dev: str = "gpu", neq: bool = True) -> Comparison:
+            judge_at: int = 1000, dev: str = "gpu", neq: bool = True,
+            mirror: bool = True) -> Comparison:
     """Cold start vs learned warm start on the same cases, both judged by the plain solver.
 
-    The warm start is the network's (rho, u) with, if neq, the non-equilibrium part of
+    The warm start is the network's (rho, u) (with mirror, averaged with the reflected
+    prediction of the reflected case) and, with neq, the non-equilibrium part of
     populations().  Both runs stop when every case is below tol.  After judge_at steps the
     warm run's residual is compared with the cold run's at the same step count; a warm start
     that is worse is rejected and that case is re-run cold, its cost counting the wasted
@@ -370,7 +388,8 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
     cold, _ = solve(cases, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
                     keep_fields=False)
     t0 = time.perf_counter()
-    pred = net.predict(features(cases))
+    pred = net.predict_cases(cases, mirror) if hasattr(net, "predict_cases") else \
+        net.predict(features(cases))
     init = [from_targets(c, p) for c, p in zip(cases, pred)]
     t_inf = time.perf_counter() - t0
     warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
