@@ -13,7 +13,9 @@ error against a double-storage run of the same kernels), D3Q27 BGK, and a CPU ba
 the oracle (OpenMP C++ restatement of the reference's kernels) on a bounded sample.
 
 roofline.achieved = 217 B per lattice update (27 float populations read and written
-once, 1 flag byte) * cells / average launch duration.
+once, 1 flag byte) * cells / average launch duration.  memory_ceilings_gbs reports what
+plain copy / read / write / in-place plane kernels moving the same bytes reach on the same
+GPU (boltzmann_amd/bw_probe); inplace27 is the ceiling for the update's access mix.
 """
 
 from __future__ import annotations
@@ -143,12 +145,17 @@ def rb_batch(steps: int = 50000):
 
 
 def cylinder(steps_marks=(1000, 5000, 20000)):
-    """Flow past a disc in a 4096 x 1024 channel, D2Q9 TRT (magic 1/4), Re = 1000 on the
-    diameter: walls top and bottom, bounce-back disc r = 64 at (1024, 512), fixed-velocity
-    inflow u = 0.05, fixed-density outflow.  Runs f64, f32, f16s and bf16s storage and
-    reports the relative max-norm of u against the f64 run at each mark, plus MLUPS."""
+    """Flow past a disc in a 4096 x 1024 channel, D2Q9 TRT, Re = 1000 on the diameter:
+    walls top and bottom, bounce-back disc r = 64 at (1024, 512), fixed-velocity inflow
+    u = 0.05, fixed-density outflow.  Runs f64, f32, f16s and bf16s storage and reports the
+    relative max-norm of u against the f64 run at each mark, plus MLUPS.
+
+    tau- is 0.52, as in the reference's vortex-street example (vortex_street.py:102).  With
+    the magic number 1/4 instead (tau- = 13.5 at tau+ = 0.5192) this flow diverges between
+    3,000 and 5,000 steps in f64 and f32 alike."""
     import boltzmann_amd
-    from boltzmann_amd.core import CellFlags, trt
+    from boltzmann_amd import Omega
+    from boltzmann_amd.core import CellFlags
     nx, ny, r, u0 = 4096, 1024, 64, 0.05
     nu = u0 * 2 * r / 1000.0
     X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
@@ -158,7 +165,8 @@ def cylinder(steps_marks=(1000, 5000, 20000)):
     out = {}
     ref = {}
     for st in ("f64", "f32", "f16s", "bf16s"):
-        sim = boltzmann_amd.Simulation("gpu", [nx, ny], trt(3 * nu + 0.5), storage=st)
+        sim = boltzmann_amd.Simulation("gpu", [nx, ny], Omega.TRT(1 / (3 * nu + 0.5), 1 / 0.52),
+                                       storage=st)
         flags = sim.cells.flags
         flags[0, :] = CellFlags.FIXED_FLUID
         flags[-1, :] = CellFlags.FIXED_FLUID_DENSITY
@@ -186,13 +194,15 @@ def cylinder(steps_marks=(1000, 5000, 20000)):
 
 
 def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
-                      width: int = 32, depth: int = 6, max_steps: int = 60000):
+                      width: int = 32, depth: int = 6, max_steps: int = 60000,
+                      train_budget: float = 420.0):
     """Learned warm start against the plain solver's time to steady state.
 
     D2Q9 TRT channels of 512 x 256 cells with 1-8 seeded obstacles (radius / half-size
     8-32), u_in in [0.02, 0.05], Re in [10, 100].  n_train cases (seeds 0..) are solved to
     the FP64 residual 1e-6 in batches; those that converge within max_steps train the
-    network (with their mirror images across the channel); n_test held-out cases
+    network (with their mirror images across the channel) for up to `epochs` epochs within
+    train_budget seconds; n_test held-out cases
     (seeds 10^6..) run cold and warm (prediction plus its Chapman-Enskog non-equilibrium
     part) to the same residual."""
     from boltzmann_amd import warmstart as ws
@@ -206,7 +216,7 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     net, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=width, depth=depth,
-                           lr=2e-3, seed=0, log=log)
+                           lr=2e-3, seed=0, log=log, time_budget=train_budget)
     t_train = time.perf_counter() - t0
     del fields
     held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
@@ -216,10 +226,27 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
     return {"train_cases": len(cases), "train_requested": n_train,
             "train_steps_median": int(np.median(steps)) if len(steps) else None,
             "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
-            "epochs": epochs, "width": width, "depth": depth,
+            "epochs": len(losses), "width": width, "depth": depth,
             "params": int(net.net.param_count),
             "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
             "held_out": s}
+
+
+def memory_ceilings():
+    """GB/s of copy / read-only / write-only / in-place plane kernels moving the bytes of one
+    256^3 D3Q27 step (boltzmann_amd/bw_probe, a child process run before this process uses
+    the GPU): what the update's access pattern can reach on this GPU."""
+    import subprocess
+    exe = os.path.join(ROOT, "boltzmann_amd", "bw_probe")
+    if not os.path.exists(exe):
+        return {"error": "boltzmann_amd/bw_probe is not built (python boltzmann_amd/build.py)"}
+    try:
+        res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return {"error": "bw_probe timed out"}
+    if res.returncode != 0:
+        return {"error": f"bw_probe exit {res.returncode}: {res.stderr.strip()[-200:]}"}
+    return json.loads(res.stdout.strip().splitlines()[-1])
 
 
 def copy_calibration(nbytes: int = 1 << 30) -> None:
@@ -249,6 +276,8 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the memory-ceiling probe (boltzmann_amd/bw_probe)")
     ap.add_argument("--rb-steps", type=int, default=50000)
     ap.add_argument("--warmstart", action="store_true",
                     help="add the learned warm start vs cold start on 512x256 channels "
@@ -264,6 +293,9 @@ def main() -> int:
         return 2
     steps = args.steps + args.steps % 2
     n = args.n
+    ceilings = None if args.no_probe else memory_ceilings()
+    if ceilings is not None:
+        print(f"[bench] memory ceilings: {ceilings}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration()
     rho, vel = tgv(n)
@@ -319,6 +351,9 @@ def main() -> int:
                      "traffic": measured_traffic() if n == 256 else None,
                      "bytes_per_lup": B_ALG["f32"]},
         "wall_mlups": round(head["wall_mlups"], 1),
+        "memory_ceilings_gbs": ceilings,
+        "frac_of_inplace_ceiling": (round(achieved / ceilings["inplace27"], 4)
+                                    if ceilings and "inplace27" in ceilings else None),
         "mass_drift": head["mass_drift"],
         "variants": variants,
     }

@@ -4,6 +4,7 @@
 
 * libbolt_hip.so   -- HIP kernels for gfx950 and the C ABI of include/bolt.h (hipcc)
 * _boltzmann*.so   -- the Python module over that ABI (g++ + pybind11)
+* bw_probe         -- memory-ceiling probe for the update's access pattern (bench.py --probe)
 
 Every HIP translation unit is compiled with packed-FP32 instructions disabled.
 """
@@ -52,6 +53,15 @@ def build_hip() -> None:
           os.path.join(CSRC, "bolt_nn.hip"), "-o", lib_path()])
 
 
+def probe_path() -> str:
+    return os.path.join(HERE, "bw_probe")
+
+
+def build_probe() -> None:
+    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], os.path.join(CSRC, "bw_probe.hip"),
+          "-o", probe_path()])
+
+
 def build_ext() -> None:
     import pybind11
     py_inc = sysconfig.get_paths()["include"]
@@ -64,6 +74,7 @@ def build_ext() -> None:
 def build() -> None:
     build_hip()
     build_ext()
+    build_probe()
 
 
 if __name__ == "__main__":

@@ -185,6 +185,12 @@ class SolveResult:
     first_residual: np.ndarray
     history: np.ndarray = None   # [checks, cases] residual at every check
     check_every: int = 0
+    steps_run: int = 0           # steps the batch ran (all cases step together)
+
+    @property
+    def seconds_per_case_step(self) -> float:
+        """Wall time of one step of one case in this batch, residual checks included."""
+        return self.seconds / max(1, self.steps_run * len(self.steps))
 
 
 def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
@@ -230,7 +236,7 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
         if stop_when_done and (steps >= 0).all():
             break
     secs = time.perf_counter() - t0
-    res = SolveResult(steps, last, secs, first, np.array(hist), check_every)
+    res = SolveResult(steps, last, secs, first, np.array(hist), check_every, done)
     if not keep_fields:
         return res, None
     fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
@@ -318,13 +324,14 @@ def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
 
 def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
           seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1, depth: int = 5,
-          mirror: bool = True, rho_weight: float = 1 / 9):
+          mirror: bool = True, rho_weight: float = 1 / 9, time_budget: float | None = None):
     """Trains the network on solver-generated steady states with the library's kernels.
     Returns the Predictor and the per-epoch mean losses (on normalised targets).
 
     mirror adds each case reflected across the channel (an exact extra sample).  rho_weight
     scales the density channel's share of the loss (its errors cost the warm start far
-    fewer steps than velocity errors).  With log,
+    fewer steps than velocity errors).  time_budget (seconds) caps the run: after the first
+    epoch the epoch count, and with it the cosine schedule, shrinks to what fits.  With log,
     training runs log_every epochs at a time (same schedule as one call) and reports each
     chunk's last epoch loss."""
     from . import Network
@@ -343,14 +350,20 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
     net.set_data(x, y)
     del x, y
     model = Predictor(net, mean, std)
-    if log is None:
+    if log is None and time_budget is None:
         return model, net.train(epochs, lr=lr, seed=seed)
     losses = []
     t0 = time.perf_counter()
-    for e0 in range(0, epochs, log_every):
-        k = min(log_every, epochs - e0)
+    e0 = 0
+    while e0 < epochs:
+        k = 1 if e0 == 0 else min(log_every, epochs - e0)
         losses.extend(net.train(k, lr=lr, seed=seed, first=e0, total=epochs).tolist())
-        log(f"epoch {e0 + k}/{epochs}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+        e0 += k
+        spent = time.perf_counter() - t0
+        if e0 == 1 and time_budget is not None:
+            epochs = max(1, min(epochs, int(time_budget / spent)))
+        if log and (e0 % log_every == 0 or e0 == epochs):
+            log(f"epoch {e0}/{epochs}: loss {losses[-1]:.5f} ({spent:.1f} s)")
     return model, np.array(losses, np.float32)
 
 
@@ -359,19 +372,29 @@ class Comparison:
     cold_steps: np.ndarray
     warm_steps: np.ndarray       # steps of the warm run, or of warm + cold after a fallback
     fallback: np.ndarray         # warm start rejected (its first residual was worse)
-    cold_seconds: float
-    warm_seconds: float          # inference + solve (+ the fallback's cold re-run if any)
+    cold_seconds: float          # the cold batch's wall time
+    warm_seconds: float          # inference + the warm batch (+ the fallback's cold re-run)
     inference_seconds: float
+    cold_case_seconds: np.ndarray = None   # per case: steps to tol x time per case-step
+    warm_case_seconds: np.ndarray = None   # the same warm, plus its share of inference
 
     def summary(self) -> dict:
+        """Over the cases the cold run brings to the tolerance.  The batch wall times run
+        until every case has converged or max_steps, so a case that never converges (an
+        unsteady flow) sets both; the per-case times do not depend on the other cases."""
         ok = self.cold_steps > 0
         ratio = self.warm_steps[ok] / self.cold_steps[ok]
-        return {"cases": int(ok.sum()), "median_step_ratio": float(np.median(ratio)),
-                "mean_step_ratio": float(np.mean(ratio)), "fallbacks": int(self.fallback.sum()),
-                "cold_seconds": round(self.cold_seconds, 3),
-                "warm_seconds": round(self.warm_seconds, 3),
-                "wall_ratio": round(self.warm_seconds / self.cold_seconds, 3),
-                "inference_seconds": round(self.inference_seconds, 3)}
+        out = {"cases": int(ok.sum()), "median_step_ratio": float(np.median(ratio)),
+               "mean_step_ratio": float(np.mean(ratio)), "fallbacks": int(self.fallback.sum()),
+               "cold_seconds": round(self.cold_seconds, 3),
+               "warm_seconds": round(self.warm_seconds, 3),
+               "batch_wall_ratio": round(self.warm_seconds / self.cold_seconds, 3),
+               "inference_seconds": round(self.inference_seconds, 3)}
+        if self.cold_case_seconds is not None:
+            w, c = self.warm_case_seconds[ok], self.cold_case_seconds[ok]
+            out["time_to_residual_ratio"] = round(float(w.sum() / c.sum()), 4)
+            out["median_case_time_ratio"] = round(float(np.median(w / c)), 4)
+        return out
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
@@ -408,6 +431,14 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
         for k, i in enumerate(idx):
             steps[i] = (j + 1) * check_every + redo.steps[k] if redo.steps[k] >= 0 else -1
         secs += redo.seconds
-    cmp = Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf)
+    # a warm run that misses the tolerance where the cold run reached it counts max_steps
+    steps = np.where((steps < 0) & (cold.steps > 0), max_steps, steps)
+    # per-case time to the tolerance: its steps at the measured cost of one case-step (the
+    # same lattice in both runs), plus its share of the inference for the warm start
+    per_cold = cold.seconds_per_case_step
+    per_warm = warm.seconds_per_case_step
+    cold_case = np.where(cold.steps > 0, cold.steps * per_cold, np.nan)
+    warm_case = np.where(steps > 0, steps * per_warm + t_inf / len(cases), np.nan)
+    cmp = Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf, cold_case, warm_case)
     cmp.warm_raw_steps = warm.steps
     return cmp
