@@ -563,6 +563,31 @@ PYBIND11_MODULE(_boltzmann, m) {
             check(bolt_lattice_residual(s.core->lat, steps_between, &r), "residual");
             return r;
         }, py::arg("steps_between"))
+        .def("postproc", [](PySimulation& s, const std::string& what, py::object origin) {
+            // utils/postproc.py:12-127 on the device velocity (lattice units)
+            std::lock_guard<std::mutex> lk(s.core->mu);
+            Core& c = *s.core;
+            if (c.iteration == 0)
+                throw BoltError("postproc reads the device velocity: call it after iterate");
+            int kind;
+            if (what == "vmag") kind = BOLT_POST_VMAG;
+            else if (what == "curl") kind = BOLT_POST_CURL;
+            else if (what == "stream") kind = BOLT_POST_STREAM;
+            else throw BoltError("unknown post-processing '" + what + "' (vmag, curl, stream)");
+            std::vector<ssize_t> shape(c.counts.begin(), c.counts.end());
+            if (kind == BOLT_POST_CURL && c.d == 3) shape.push_back(3);
+            py::array_t<float> out(shape);
+            int o[2] = {0, 0};
+            if (!origin.is_none()) {
+                auto t = origin.cast<std::vector<int>>();
+                if (t.size() != 2) throw BoltError("origin must be (x, y)");
+                o[0] = t[0];
+                o[1] = t[1];
+            }
+            check(bolt_lattice_postproc(c.lat, kind, o, out.mutable_data(), (size_t)out.size() * 4),
+                  "postproc");
+            return out;
+        }, py::arg("what"), py::arg("origin") = py::none())
         .def("set_block", [](PySimulation& s, int b) { check(bolt_lattice_set_block(s.core->lat, b), "set_block"); })
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;

@@ -1020,6 +1020,160 @@ int bolt_lattice_residual(bolt_lattice* L, uint64_t steps_between, double* out) 
     return BOLT_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Post-processing of the device velocity (utils/postproc.py:12-127), in float like the
+// reference's numpy on the host's float32 arrays: np.gradient (central differences inside,
+// one-sided first-order at the two ends of an axis) and sequential float32 cumsum.
+// ---------------------------------------------------------------------------------------
+template <typename R>
+__device__ inline float velf(const R* vel, long long ncell, int a, long long c) {
+    return (float)vel[(long long)a * ncell + c];
+}
+
+// d/dx_axis of component a at cell (i0, i1, i2) of counts n, strides st (np.gradient)
+template <typename R>
+__device__ inline float grad(const R* vel, long long ncell, int a, const int* n, const long long* st,
+                             const int* idx, int axis, long long c) {
+    const int i = idx[axis];
+    const long long s = st[axis];
+    if (i == 0) return velf(vel, ncell, a, c + s) - velf(vel, ncell, a, c);
+    if (i == n[axis] - 1) return velf(vel, ncell, a, c) - velf(vel, ncell, a, c - s);
+    return (velf(vel, ncell, a, c + s) - velf(vel, ncell, a, c - s)) / 2.0f;
+}
+
+template <typename R>
+__global__ void k_vmag(const R* vel, long long ncell, int d, float* out) {
+#pragma clang fp contract(off)  // numpy rounds u*u before the sum: no fused multiply-add
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    float s = 0.f;
+    for (int a = 0; a < d; ++a) {
+        const float u = velf(vel, ncell, a, c);
+        s = s + u * u;
+    }
+    // correctly rounded float sqrt, as numpy's: the double square root rounded to float
+    out[c] = (float)sqrt((double)s);
+}
+
+template <typename R>
+__global__ void k_curl(const R* vel, long long ncell, int d, int n0, int n1, int n2, float* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int n[3] = {n0, n1, n2};
+    const long long st[3] = {(long long)n1 * n2, (long long)n2, 1};
+    const int idx[3] = {(int)(c / st[0]), (int)((c / n2) % n1), (int)(c % n2)};
+    if (d == 2) {
+        // du/dy - dv/dx (postproc.py: grad[1][..., 0] - grad[0][..., 1])
+        out[c] = grad(vel, ncell, 0, n, st, idx, 1, c) - grad(vel, ncell, 1, n, st, idx, 0, c);
+    } else {
+        out[3 * c + 0] = grad(vel, ncell, 1, n, st, idx, 2, c) - grad(vel, ncell, 2, n, st, idx, 1, c);
+        out[3 * c + 1] = grad(vel, ncell, 2, n, st, idx, 0, c) - grad(vel, ncell, 0, n, st, idx, 2, c);
+        out[3 * c + 2] = grad(vel, ncell, 0, n, st, idx, 1, c) - grad(vel, ncell, 1, n, st, idx, 0, c);
+    }
+}
+
+// ivdx = cumsum(v, axis=0): one thread per y walks x (coalesced across threads)
+template <typename R>
+__global__ void k_cumsum_v_x(const R* vel, long long ncell, int nx, int ny, float* ivdx) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ny) return;
+    float acc = 0.f;
+    for (int x = 0; x < nx; ++x) {
+        const long long c = (long long)x * ny + y;
+        acc = acc + velf(vel, ncell, 1, c);
+        ivdx[c] = acc;
+    }
+}
+
+// iudy[0, :] = cumsum(u[0, :]): one work-item walks row x = 0
+template <typename R>
+__global__ void k_cumsum_u_row0(const R* vel, long long ncell, int ny, float* iudy0) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float acc = 0.f;
+    for (int y = 0; y < ny; ++y) {
+        acc = acc + velf(vel, ncell, 0, y);
+        iudy0[y] = acc;
+    }
+}
+
+// phi = iudy[0, :] - ivdx, then phi -= phi[origin]
+__global__ void k_stream_combine(const float* ivdx, const float* iudy0, long long ncell, int ny,
+                                 int ox, int oy, float* phi) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const long long o = (long long)ox * ny + oy;
+    const float p0 = iudy0[oy] - ivdx[o];
+    const float p = iudy0[c % ny] - ivdx[c];
+    phi[c] = p - p0;
+}
+
+int bolt_lattice_postproc(bolt_lattice* L, int kind, const int* origin, float* host, size_t nbytes) {
+    if (!L || !host) return fail(BOLT_EINVAL, "null argument");
+    const int d = L->ndim;
+    size_t per = 1;
+    if (kind == BOLT_POST_CURL && d == 3) per = 3;
+    if (kind == BOLT_POST_STREAM && d != 2)
+        return fail(BOLT_EINVAL, "the stream function is defined for 2D lattices only");
+    if (kind != BOLT_POST_VMAG && kind != BOLT_POST_CURL && kind != BOLT_POST_STREAM)
+        return fail(BOLT_EINVAL, "unknown post-processing kind %d", kind);
+    if (kind != BOLT_POST_VMAG)
+        for (int a = 0; a < d; ++a)
+            if (L->n[a] < 2) return fail(BOLT_EINVAL, "gradients need at least 2 cells along every axis");
+    const size_t want = (size_t)L->ncell * per * sizeof(float);
+    if (nbytes != want) return fail(BOLT_EINVAL, "expected %zu bytes, got %zu", want, nbytes);
+    int ox = 0, oy = 0;
+    if (kind == BOLT_POST_STREAM && origin) {
+        ox = origin[0];
+        oy = origin[1];
+        if (ox < 0 || ox >= L->n[0] || oy < 0 || oy >= L->n[1])
+            return fail(BOLT_EINVAL, "origin (%d, %d) outside the lattice", ox, oy);
+    }
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    hipStream_t s = L->ctx->stream;
+    // scratch: the result (ncell * per floats) and, for the stream function, ivdx and row 0
+    const size_t need = (size_t)L->ncell * per + (kind == BOLT_POST_STREAM ? (size_t)L->ncell + L->n[1] : 0);
+    float* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, need * sizeof(float)));
+    const unsigned blocks = (unsigned)((L->ncell + 255) / 256);
+    const bool f64 = is_f64(L);
+    const void* v = L->vel;
+    if (kind == BOLT_POST_VMAG) {
+        if (f64) hipLaunchKernelGGL(k_vmag<double>, dim3(blocks), dim3(256), 0, s, (const double*)v, L->ncell, d, buf);
+        else hipLaunchKernelGGL(k_vmag<float>, dim3(blocks), dim3(256), 0, s, (const float*)v, L->ncell, d, buf);
+    } else if (kind == BOLT_POST_CURL) {
+        if (f64)
+            hipLaunchKernelGGL(k_curl<double>, dim3(blocks), dim3(256), 0, s, (const double*)v, L->ncell, d, L->n[0],
+                               L->n[1], L->n[2], buf);
+        else
+            hipLaunchKernelGGL(k_curl<float>, dim3(blocks), dim3(256), 0, s, (const float*)v, L->ncell, d, L->n[0],
+                               L->n[1], L->n[2], buf);
+    } else {
+        float* phi = buf;
+        float* ivdx = buf + L->ncell;
+        float* iudy0 = ivdx + L->ncell;
+        const unsigned yb = (unsigned)((L->n[1] + 255) / 256);
+        if (f64) {
+            hipLaunchKernelGGL(k_cumsum_v_x<double>, dim3(yb), dim3(256), 0, s, (const double*)v, L->ncell, L->n[0],
+                               L->n[1], ivdx);
+            hipLaunchKernelGGL(k_cumsum_u_row0<double>, dim3(1), dim3(64), 0, s, (const double*)v, L->ncell, L->n[1],
+                               iudy0);
+        } else {
+            hipLaunchKernelGGL(k_cumsum_v_x<float>, dim3(yb), dim3(256), 0, s, (const float*)v, L->ncell, L->n[0],
+                               L->n[1], ivdx);
+            hipLaunchKernelGGL(k_cumsum_u_row0<float>, dim3(1), dim3(64), 0, s, (const float*)v, L->ncell, L->n[1],
+                               iudy0);
+        }
+        hipLaunchKernelGGL(k_stream_combine, dim3(blocks), dim3(256), 0, s, ivdx, iudy0, L->ncell, L->n[1], ox, oy,
+                           phi);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, buf, want, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(buf);
+    if (e != hipSuccess) return fail(BOLT_EDEVICE, "postproc: %s", hipGetErrorString(e));
+    return BOLT_OK;
+}
+
 // =======================================================================================
 // Batches: many independent small D2Q9 + D2Q5 Boussinesq lattices advanced together.
 //

@@ -8,7 +8,8 @@
 //   inplace27     27 float planes read and written back in place (the update's mix)
 //   inplace27_h   the same with 2-byte elements (the f16s/bf16s storage)
 //   inplace9      9 float planes in place over 3x the cells (the D2Q9 mix)
-// and prints one JSON object of GB/s (bytes read + written per second).  Every index is
+// (each in-place figure the better of plain and streaming loads) and prints one JSON
+// object of GB/s (bytes read + written per second).  Every index is
 // checked against the element count it addresses.
 #include <hip/hip_runtime.h>
 
@@ -48,13 +49,17 @@ __global__ void write_planes(float* __restrict__ f, long long n) {
     for (int q = 0; q < Q; ++q) f[q * n + i] = (float)q;
 }
 
-template <int Q, typename T>
+// NT: streaming (non-temporal) loads, as the update kernels use
+template <int Q, typename T, bool NT>
 __global__ void inplace_planes(T* __restrict__ f, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     T v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = f[q * n + i];
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (NT) v[q] = __builtin_nontemporal_load(f + q * n + i);
+        else v[q] = f[q * n + i];
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q * n + i] = v[(q + 1) % Q];
 }
@@ -97,12 +102,19 @@ int main(int argc, char** argv) {
     double r_copy = gbs([&] { copy4<<<grid(n4), b>>>((const float4*)f, (float4*)g, n4); }, lat, reps);
     double r_read = gbs([&] { read_planes<27><<<grid(n), b>>>(f, sink, n); }, 4.0 * total, reps);
     double r_write = gbs([&] { write_planes<27><<<grid(n), b>>>(f, n); }, 4.0 * total, reps);
-    double r_in = gbs([&] { inplace_planes<27, float><<<grid(n), b>>>(f, n); }, lat, reps);
+    // the in-place mixes, each the better of plain and streaming loads
+    auto best = [](double a, double c) { return a > c ? a : c; };
+    double r_in = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n); }, lat, reps),
+                       gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n); }, lat, reps));
     // 2-byte elements: 27 planes of n halves occupy half of the first lattice
-    double r_inh = gbs([&] { inplace_planes<27, uint16_t><<<grid(n), b>>>((uint16_t*)f, n); },
-                       2.0 * 2.0 * (double)total, reps);
+    const double half = 2.0 * 2.0 * (double)total;
+    double r_inh = best(
+        gbs([&] { inplace_planes<27, uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps),
+        gbs([&] { inplace_planes<27, uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps));
     // 9 planes over 3n cells: the same 27n floats
-    double r_in9 = gbs([&] { inplace_planes<9, float><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps);
+    double r_in9 = best(
+        gbs([&] { inplace_planes<9, float, false><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps),
+        gbs([&] { inplace_planes<9, float, true><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps));
 
     hipDeviceProp_t p;
     CK(hipGetDeviceProperties(&p, 0));

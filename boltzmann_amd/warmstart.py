@@ -199,8 +199,10 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
 
     init: None for the cold start, else a list of (rho, vel) per case; with neq the
     initial populations carry the non-equilibrium part of populations(), else they are at
-    equilibrium.  Returns the SolveResult and, with keep_fields, the final (rho, vel) of
-    every case."""
+    equilibrium.  SolveResult.seconds covers the set-up (state upload, initial
+    populations) and the stepping.  Returns the SolveResult and, with keep_fields, the
+    final (rho, vel) of every case."""
+    t0 = time.perf_counter()
     b = batch or _batch(cases, dev)
     written = []
     for k, c in enumerate(cases):
@@ -217,7 +219,6 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
     last = np.full(n, np.inf)
     first = np.full(n, np.nan)
     hist = []
-    t0 = time.perf_counter()
     done = 0
     while done < max_steps:
         b.step(check_every - 100)
@@ -265,6 +266,14 @@ def features(cases) -> np.ndarray:
     return np.stack([c.features().transpose(1, 2, 0) for c in cases]).astype(np.float32)
 
 
+def mirrored_features(x: np.ndarray) -> np.ndarray:
+    """features() of the reflected cases from those of the cases: every channel reflects
+    with the geometry except y / ny (channel 5), which belongs to the lattice."""
+    m = np.ascontiguousarray(x[:, :, ::-1])
+    m[..., 5] = x[..., 5]
+    return m
+
+
 def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
              check_every: int = 2000, tol: float = 1e-6, log=None, **case_kw):
     """Solves n cases (seeds seed0 ..) to the tolerance in batches; returns the converged
@@ -301,12 +310,11 @@ class Predictor:
         """Targets for cases; with mirror, the mean of the prediction and the reflected
         prediction of the reflected case (exact symmetry, so only the network's error
         differs between the two)."""
-        t = self.predict(features(cases))
+        x = features(cases)
+        t = self.predict(x)
         if not mirror:
             return t
-        mc = [Case(c.seed, c.nx, c.ny, c.u_in, c.re, c.tau, np.ascontiguousarray(c.mask[:, ::-1]))
-              for c in cases]
-        tm = self.predict(features(mc))[:, :, ::-1].copy()
+        tm = self.predict(mirrored_features(x))[:, :, ::-1].copy()
         tm[..., 1] *= -1
         return 0.5 * (t + tm)
 
@@ -398,13 +406,17 @@ class Comparison:
 
 
 def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: int = 100000,
-            judge_at: int = 1000, dev: str = "gpu", neq: bool = True,
+            judge_at: int = 1000, dev: str = "gpu", neq: bool = False,
             mirror: bool = True) -> Comparison:
     """Cold start vs learned warm start on the same cases, both judged by the plain solver.
 
     The warm start is the network's (rho, u) (with mirror, averaged with the reflected
     prediction of the reflected case) and, with neq, the non-equilibrium part of
-    populations().  Both runs stop when every case is below tol.  After judge_at steps the
+    populations().  neq pays when the prediction is very close (a converged state restarts
+    in a third of the steps); at the network's present accuracy it changes the step count
+    by under 2 % and costs host time, so it is off by default.  Both runs are timed from
+    the set-up of their batch; the warm run also counts feature construction and
+    inference.  Both runs stop when every case is below tol.  After judge_at steps the
     warm run's residual is compared with the cold run's at the same step count; a warm start
     that is worse is rejected and that case is re-run cold, its cost counting the wasted
     warm steps."""

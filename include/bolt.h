@@ -67,6 +67,11 @@ extern "C" {
 #define BOLT_REDUCE_MOMENTUM 1  /* sum over non-wall cells of rho*u: D values          */
 #define BOLT_REDUCE_TRACER 2    /* sum of every tracer population slot: 1 value       */
 
+/* post-processing of the velocity (bolt_lattice_postproc) */
+#define BOLT_POST_VMAG 0    /* |u| per cell                                              */
+#define BOLT_POST_CURL 1    /* 2D: du/dy - dv/dx per cell; 3D: 3 components per cell      */
+#define BOLT_POST_STREAM 2  /* 2D stream function, zero at the origin cell               */
+
 typedef struct bolt_ctx bolt_ctx;
 typedef struct bolt_lattice bolt_lattice;
 
@@ -144,6 +149,18 @@ int bolt_lattice_reduce(bolt_lattice* lat, int what, int tracer_id, double* out)
  * first call only saves the snapshot and returns +inf.  Reads the velocity of the last
  * completed bolt_lattice_step. */
 int bolt_lattice_residual(bolt_lattice* lat, uint64_t steps_between, double* out);
+
+/* Post-processing of the device velocity into a host float array, with the semantics of
+ * python/boltzmann/utils/postproc.py:12-127 on the float32 velocity (np.gradient: central
+ * differences inside, one-sided at the ends of each axis; sequential float32 cumsum), in
+ * lattice units (the caller applies unit scales):
+ *   BOLT_POST_VMAG    calc_vmag: sqrt(sum_a u_a^2), [cells]
+ *   BOLT_POST_CURL    calc_curl: 2D du/dy - dv/dx [cells]; 3D [cells][3]
+ *   BOLT_POST_STREAM  calc_stream_func (2D): cumsum(u[0, :]) - cumsum(v, axis 0), minus its
+ *                     value at origin[2] (NULL: cell (0, 0)), [cells]
+ * Reads the velocity of the last completed bolt_lattice_step or bolt_lattice_write. */
+int bolt_lattice_postproc(bolt_lattice* lat, int kind, const int* origin, float* host,
+                          size_t nbytes);
 
 /* Device bytes held by the lattice's buffers. */
 int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);

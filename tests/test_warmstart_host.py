@@ -142,3 +142,10 @@ def test_mirrored_case_evolves_as_the_mirror_image():
     assert np.abs(v0).max() > 1e-3                       # the flow has developed
     np.testing.assert_allclose(r1, rm, rtol=0, atol=1e-6)
     np.testing.assert_allclose(v1, vm, rtol=0, atol=1e-6)
+
+
+def test_mirrored_features_equal_the_reflected_cases_features():
+    cases = [ws.make_case(7 + s, nx=128, ny=64, size_range=(4, 10)) for s in range(3)]
+    x = ws.features(cases)
+    reflected = [ws.mirrored(c, *c.cold_state())[0] for c in cases]
+    np.testing.assert_array_equal(ws.mirrored_features(x), ws.features(reflected))
