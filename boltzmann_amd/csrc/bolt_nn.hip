@@ -137,6 +137,14 @@ static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, con
 #undef BOLT_GEMM
 }
 
+// dbias += column sums of dz [P][C]: up to 1024 work-groups of contiguous row chunks
+static void bias_grad_launch(bolt_nn* N, long long P, int C, float* db) {
+    const long long groups = std::min<long long>(1024, std::max<long long>(1, P / 64));
+    const long long rows = (P + groups - 1) / groups;
+    hipLaunchKernelGGL(bias_grad, dim3((unsigned)((P + rows - 1) / rows)), dim3(256), 0, N->ctx->stream,
+                       N->dz, P, C, rows, db);
+}
+
 // forward 3x3 conv of X (channel count cin, all channels) into Y (ldy, channel offset yoff)
 static void conv_fwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, u16* Y, int ldy, int yoff) {
     hipStream_t s = N->ctx->stream;
@@ -157,7 +165,7 @@ static void conv_bwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, co
     const long long P = (long long)N->batch * nx * ny;
     hipLaunchKernelGGL(relu_grad, dim3(nblk(P * L.cout)), dim3(256), 0, s, dY, lddy, dyoff, Y, ldy, yoff,
                        N->dz, P, L.cout, L.relu ? 1 : 0);
-    hipLaunchKernelGGL(bias_grad, dim3(L.cout), dim3(256), 0, s, N->dz, P, L.cout, N->G + L.b_off);
+    bias_grad_launch(N, P, L.cout, N->G + L.b_off);
     hipLaunchKernelGGL(im2col3x3, dim3(nblk(P * 9 * (L.cin / 8))), dim3(256), 0, s, X, L.cin, 0, N->col,
                        N->batch, nx, ny, L.cin);
     // dW[co][j] += sum_p dz[p][co] col[p][j]
@@ -224,7 +232,7 @@ static void backward(bolt_nn* N) {
     const int w0 = O.cin;
     hipLaunchKernelGGL(relu_grad, dim3(nblk(P0 * kOutC)), dim3(256), 0, s, N->dout, kOutC, 0, (const u16*)nullptr,
                        0, 0, N->dz, P0, kOutC, 0);
-    hipLaunchKernelGGL(bias_grad, dim3(kOutC), dim3(256), 0, s, N->dz, P0, kOutC, N->G + O.b_off);
+    bias_grad_launch(N, P0, kOutC, N->G + O.b_off);
     gemm(N, kKM, kKN, kAtomicF32, N->dz, kOutC, N->u[0], w0, N->G + O.w_off, w0, 0, kOutC, w0, (int)P0, nullptr,
          false, 4096);
     gemm(N, kMK, kKN, kStoreF32, N->dz, kOutC, N->Wb + O.w_off, w0, N->du[0], w0, 0, (int)P0, w0, kOutC, nullptr,
@@ -320,7 +328,8 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int dept
     if (depth < 1 || depth > 8) return fail(BOLT_EINVAL, "depth must be 1..8");
     const int m = 1 << depth;
     if (nx % m || ny % m) return fail(BOLT_EINVAL, "nx and ny must be multiples of 2^depth = %d", m);
-    if (width < 16 || width % 16) return fail(BOLT_EINVAL, "width must be a multiple of 16");
+    if (width < 16 || width % 16 || width > 64)
+        return fail(BOLT_EINVAL, "width must be a multiple of 16 from 16 to 64");
     if (batch < 1 || batch > 64) return fail(BOLT_EINVAL, "batch must be 1..64");
     if ((long long)batch * nx * ny > (1ll << 26)) return fail(BOLT_EINVAL, "batch * nx * ny too large");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -416,9 +425,10 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int dept
         if (!c.relu)
             for (size_t i = 0; i < (size_t)c.cout * c.taps * c.cin; ++i) h[c.w_off + i] *= 0.1f;
     }
-    HIP_TRY(hipMemcpy(n->W, h.data(), off * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(n->M1, 0, off * 4));
-    HIP_TRY(hipMemset(n->M2, 0, off * 4));
+    // everything on the context's non-blocking stream: the null stream orders nothing here
+    HIP_TRY(hipMemcpyAsync(n->W, h.data(), off * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(n->M1, 0, off * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(n->M2, 0, off * 4, ctx->stream));
     hipLaunchKernelGGL(to_bf16, dim3(nblk((long long)off)), dim3(256), 0, ctx->stream, n->W, n->Wb, (long long)off);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -435,14 +445,15 @@ int bolt_nn_param_count(const bolt_nn* N, uint64_t* out) {
 int bolt_nn_get_params(bolt_nn* N, float* out, uint64_t n) {
     if (!N || !out || n != N->nparam) return fail(BOLT_EINVAL, "expected %zu parameters", N ? N->nparam : 0);
     HIP_TRY(hipSetDevice(N->ctx->device));
-    HIP_TRY(hipMemcpy(out, N->W, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(out, N->W, n * 4, hipMemcpyDeviceToHost, N->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(N->ctx->stream));
     return BOLT_OK;
 }
 
 int bolt_nn_set_params(bolt_nn* N, const float* in, uint64_t n) {
     if (!N || !in || n != N->nparam) return fail(BOLT_EINVAL, "expected %zu parameters", N ? N->nparam : 0);
     HIP_TRY(hipSetDevice(N->ctx->device));
-    HIP_TRY(hipMemcpy(N->W, in, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(N->W, in, n * 4, hipMemcpyHostToDevice, N->ctx->stream));
     hipLaunchKernelGGL(to_bf16, dim3(nblk((long long)n)), dim3(256), 0, N->ctx->stream, N->W, N->Wb, (long long)n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(N->ctx->stream));
@@ -462,7 +473,10 @@ int bolt_nn_loss_grad(bolt_nn* N, const float* x, const float* y, double* loss_o
     backward(N);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(N->ctx->stream));
-    if (grads) HIP_TRY(hipMemcpy(grads, N->G, ng * 4, hipMemcpyDeviceToHost));
+    if (grads) {
+        HIP_TRY(hipMemcpyAsync(grads, N->G, ng * 4, hipMemcpyDeviceToHost, N->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(N->ctx->stream));
+    }
     return BOLT_OK;
 }
 
@@ -481,9 +495,12 @@ int bolt_nn_set_data(bolt_nn* N, int n, const float* x, const float* y) {
     std::vector<uint16_t> xb;
     for (int i = 0; i < n; ++i) {
         pad_features(x + (size_t)i * pix * kFeat, pix, xb);
-        HIP_TRY(hipMemcpy(N->data_x + (size_t)i * pix * kInC, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(N->data_x + (size_t)i * pix * kInC, xb.data(), xb.size() * 2,
+                               hipMemcpyHostToDevice, N->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(N->ctx->stream));  // xb is refilled next iteration
     }
-    HIP_TRY(hipMemcpy(N->data_y, y, (size_t)n * pix * kOutC * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(N->data_y, y, (size_t)n * pix * kOutC * 4, hipMemcpyHostToDevice, N->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(N->ctx->stream));
     N->data_n = n;
     return BOLT_OK;
 }
@@ -582,15 +599,19 @@ int bolt_nn_gemm_check(bolt_ctx* ctx, int a_form, int b_form, int M, int Nn, int
     if (hipMalloc(&da, ha.size() * 2) || hipMalloc(&db, hb.size() * 2) || hipMalloc(&dc, (size_t)M * Nn * 4)) {
         rc = fail(BOLT_ENOMEM, "gemm check allocation failed");
     } else {
-        hipMemcpy(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice);
-        hipMemcpy(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice);
-        hipMemset(dc, 0, (size_t)M * Nn * 4);
+        // stream-ordered before the GEMM (the context stream does not wait for the null
+        // stream; a late zero-fill would wipe split-K partial sums)
+        hipStream_t s = ctx->stream;
+        hipMemcpyAsync(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice, s);
+        hipMemsetAsync(dc, 0, (size_t)M * Nn * 4, s);
         const int lda = a_form == kMK ? K : M, ldb = b_form == kNK ? K : Nn;
         gemm(&tmp, a_form, b_form, k_split > 0 ? kAtomicF32 : kStoreF32, da, lda, db, ldb, dc, Nn, 0, M, Nn, K,
              nullptr, false, k_split);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(c, dc, (size_t)M * Nn * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpyAsync(c, dc, (size_t)M * Nn * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = fail(BOLT_EDEVICE, "gemm check: %s", hipGetErrorString(e));
     }
     hipFree(da);

@@ -1281,7 +1281,10 @@ static int batch_build_tiles(bolt_batch* B, int block) {
     hipFree(B->d_tiles);
     B->d_tiles = nullptr;
     HIP_TRY(hipMalloc(&B->d_tiles, tiles.size() * sizeof(Tile)));
-    HIP_TRY(hipMemcpy(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    // on the batch's (non-blocking) stream, so the launches that read the table follow it
+    HIP_TRY(hipMemcpyAsync(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                           B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     B->ntiles = (int)tiles.size();
     B->block = block;
     batch_drop_graph(B);
@@ -1371,7 +1374,9 @@ static int batch_upload_desc(bolt_batch* B) {
     hipFree(B->d_inst);
     B->d_inst = nullptr;
     HIP_TRY(hipMalloc(&B->d_inst, h.size() * sizeof(InstDesc)));
-    HIP_TRY(hipMemcpy(B->d_inst, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(B->d_inst, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice,
+                           B->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     return BOLT_OK;
 }
 

@@ -180,19 +180,25 @@ __global__ void relu_grad(const float* __restrict__ dY, int ldy, int yoff, const
     dZ[t] = bf16_bits(g);
 }
 
-// dbias[c] += sum_p dZ[p][c]
-__global__ void bias_grad(const u16* __restrict__ dZ, long long P, int C, float* __restrict__ db) {
-    const int c = blockIdx.x;
+// dbias[c] += sum_p dZ[p][c] (C <= 256, 256 threads).  Each work-group sums rows
+// [p0, p0 + rows) with contiguous loads: with the stride S = (256 / C) * C, a multiple of
+// C, thread t only ever sees channel t % C; then one atomic add per channel and group.
+__global__ void bias_grad(const u16* __restrict__ dZ, long long P, int C, long long rows,
+                          float* __restrict__ db) {
+    const int S = (int)(blockDim.x / C) * C;
+    const long long p0 = (long long)blockIdx.x * rows;
+    const long long p1 = p0 + rows < P ? p0 + rows : P;
     float s = 0.f;
-    for (long long p = threadIdx.x; p < P; p += blockDim.x) s += bf16_float(dZ[p * C + c]);
+    if ((int)threadIdx.x < S)
+        for (long long e = p0 * C + threadIdx.x; e < p1 * C; e += S) s += bf16_float(dZ[e]);
     __shared__ float sh[256];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
-        __syncthreads();
+    if ((int)threadIdx.x < C) {
+        float t = 0.f;
+        for (int j = threadIdx.x; j < S; j += C) t += sh[j];
+        atomicAdd(&db[threadIdx.x], t);
     }
-    if (threadIdx.x == 0) db[c] += sh[0];
 }
 
 // 2x2 average pooling (channels-last), ld/off select a channel slice of X
