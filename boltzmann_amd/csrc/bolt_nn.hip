@@ -1,5 +1,5 @@
 // The warm-start network: a small U-Net trained and run with this library's own kernels
-// (nn_kernels.hpp: bf16 MFMA GEMM, im2col/col2im, element-wise layers).
+// (nn_kernels.hpp: bf16 MFMA GEMM with implicit 3x3 convolution operands, element-wise layers).
 //
 // Input  [batch][nx][ny][16]  bf16 (the caller's 8 feature channels, zero-padded to 16)
 // Output [batch][nx][ny][3]   float
@@ -78,8 +78,7 @@ struct bolt_nn {
     float *dout = nullptr, *dbb = nullptr;
     std::vector<float*> du, dcat, dpin, dea;
     u16* dz;       // bf16 pre-activation gradient of the layer being processed
-    u16* col;      // im2col scratch
-    float* dcol;   // input-gradient columns
+    u16* wt;       // reflected, transposed kernel of the layer being processed (input gradient)
     float* target;
     double* partial;
     // dataset on the device
@@ -105,8 +104,9 @@ static int nn_alloc(bolt_nn* N, void** p, size_t bytes) {
 // ---- launch helpers ---------------------------------------------------------------------
 static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, const u16* b, int ldb,
                  void* c, int ldc, int c_off, int M, int Nn, int K, const float* bias, bool relu,
-                 int k_split = 0) {
+                 int k_split = 0, ConvGeom cg = {1, 1, 8}) {
     GemmArgs g;
+    g.conv = cg;
     g.a = a;
     g.b = b;
     g.c = c;
@@ -132,7 +132,9 @@ static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, con
     if (af == kMK && bf == kNK) BOLT_GEMM_OUT(kMK, kNK);
     else if (af == kMK && bf == kKN) BOLT_GEMM_OUT(kMK, kKN);
     else if (af == kKM && bf == kNK) BOLT_GEMM_OUT(kKM, kNK);
-    else BOLT_GEMM_OUT(kKM, kKN);
+    else if (af == kKM && bf == kKN) BOLT_GEMM_OUT(kKM, kKN);
+    else if (af == kMKconv && bf == kNK) BOLT_GEMM_OUT(kMKconv, kNK);
+    else BOLT_GEMM_OUT(kKM, kKNconv);
 #undef BOLT_GEMM_OUT
 #undef BOLT_GEMM
 }
@@ -145,16 +147,13 @@ static void bias_grad_launch(bolt_nn* N, long long P, int C, float* db) {
                        N->dz, P, C, rows, db);
 }
 
-// forward 3x3 conv of X (channel count cin, all channels) into Y (ldy, channel offset yoff)
+// forward 3x3 conv of X (channel count cin, all channels) into Y (ldy, channel offset yoff):
+// an implicit GEMM whose A operand gathers the 3x3 neighbourhoods of X
 static void conv_fwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, u16* Y, int ldy, int yoff) {
-    hipStream_t s = N->ctx->stream;
     const long long P = (long long)N->batch * nx * ny;
-    if (L.taps == 9) {
-        hipLaunchKernelGGL(im2col3x3, dim3(nblk(P * 9 * (L.cin / 8))), dim3(256), 0, s, X, L.cin, 0,
-                           N->col, N->batch, nx, ny, L.cin);
-        gemm(N, kMK, kNK, kStoreBF16, N->col, 9 * L.cin, N->Wb + L.w_off, 9 * L.cin, Y, ldy, yoff,
-             (int)P, L.cout, 9 * L.cin, N->W + L.b_off, L.relu);
-    }
+    if (L.taps == 9)
+        gemm(N, kMKconv, kNK, kStoreBF16, X, L.cin, N->Wb + L.w_off, 9 * L.cin, Y, ldy, yoff, (int)P, L.cout,
+             9 * L.cin, N->W + L.b_off, L.relu, 0, ConvGeom{nx, ny, L.cin});
 }
 
 // backward of a 3x3 conv whose output was written at (Y, ldy, yoff) with gradient (dY, lddy,
@@ -166,17 +165,16 @@ static void conv_bwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, co
     hipLaunchKernelGGL(relu_grad, dim3(nblk(P * L.cout)), dim3(256), 0, s, dY, lddy, dyoff, Y, ldy, yoff,
                        N->dz, P, L.cout, L.relu ? 1 : 0);
     bias_grad_launch(N, P, L.cout, N->G + L.b_off);
-    hipLaunchKernelGGL(im2col3x3, dim3(nblk(P * 9 * (L.cin / 8))), dim3(256), 0, s, X, L.cin, 0, N->col,
-                       N->batch, nx, ny, L.cin);
-    // dW[co][j] += sum_p dz[p][co] col[p][j]
-    gemm(N, kKM, kKN, kAtomicF32, N->dz, L.cout, N->col, 9 * L.cin, N->G + L.w_off, 9 * L.cin, 0,
-         L.cout, 9 * L.cin, (int)P, nullptr, false, 4096);
+    // dW[co][tap*cin + ci] += sum_p dz[p][co] X[p + shift(tap)][ci]  (B gathers X)
+    gemm(N, kKM, kKNconv, kAtomicF32, N->dz, L.cout, X, L.cin, N->G + L.w_off, 9 * L.cin, 0, L.cout,
+         9 * L.cin, (int)P, nullptr, false, 4096, ConvGeom{nx, ny, L.cin});
     if (dX) {
-        // dcol[p][j] = sum_co dz[p][co] W[co][j]; dX = col2im(dcol)
-        gemm(N, kMK, kKN, kStoreF32, N->dz, L.cout, N->Wb + L.w_off, 9 * L.cin, N->dcol, 9 * L.cin, 0,
-             (int)P, 9 * L.cin, L.cout, nullptr, false);
-        hipLaunchKernelGGL(col2im3x3, dim3(nblk(P * L.cin)), dim3(256), 0, s, N->dcol, dX, L.cin, 0, 0,
-                           N->batch, nx, ny, L.cin);
+        // dX[p][ci] = sum_tap sum_co dz[p + shift(tap)][co] W[co][(8 - tap)*cin + ci]: the
+        // forward form on dz with the reflected, transposed kernel Wt
+        const long long nw = (long long)L.cout * 9 * L.cin;
+        hipLaunchKernelGGL(flip_weights, dim3(nblk(nw)), dim3(256), 0, s, N->Wb + L.w_off, L.cout, L.cin, N->wt);
+        gemm(N, kMKconv, kNK, kStoreF32, N->dz, L.cout, N->wt, 9 * L.cout, dX, L.cin, 0, (int)P, L.cin,
+             9 * L.cout, nullptr, false, 0, ConvGeom{nx, ny, L.cout});
     }
 }
 
@@ -405,15 +403,14 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int dept
     A((void**)&n->out, P0 * kOutC * 4);
     A((void**)&n->target, P0 * kOutC * 4);
     A((void**)&n->dout, P0 * kOutC * 4);
-    // scratch sized for the widest layer: max over layers of P * cout and P * 9 * cin
-    size_t zmax = 0, colmax = 0;
+    // scratch sized for the widest layer: max over layers of P * cout and of the kernel size
+    size_t zmax = 0, wmax = 0;
     for (size_t i = 0; i < N->conv.size(); ++i) {
         zmax = std::max(zmax, Ps[i] * (size_t)N->conv[i].cout);
-        colmax = std::max(colmax, Ps[i] * 9 * (size_t)N->conv[i].cin);
+        wmax = std::max(wmax, (size_t)N->conv[i].cout * N->conv[i].taps * N->conv[i].cin);
     }
     A((void**)&n->dz, zmax * 2);
-    A((void**)&n->col, colmax * 2);
-    A((void**)&n->dcol, colmax * 4);
+    A((void**)&n->wt, wmax * 2);
     A((void**)&n->partial, 256 * sizeof(double));
     if (rc) return rc;
     // He-normal initialisation, zero biases

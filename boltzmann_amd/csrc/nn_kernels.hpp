@@ -1,14 +1,17 @@
-// Kernels of the warm-start network: a bf16 MFMA GEMM with LDS staging, im2col / col2im for
-// 3x3 convolutions, and the element-wise layers.  Activations are channels-last bf16
-// ([batch][nx][ny][C], C contiguous); gradients and master weights are float.
+// Kernels of the warm-start network: a bf16 MFMA GEMM with LDS staging that also reads 3x3
+// convolution operands implicitly, and the element-wise layers.  Activations are
+// channels-last bf16 ([batch][nx][ny][C], C contiguous); gradients and master weights are
+// float.
 //
 // The GEMM computes C(m, n) = sum_k A(m, k) B(n, k) with
-//   A in "MK" form (A(m,k) at a[m*lda + k]) or "KM" form (a[k*lda + m]),
-//   B in "NK" form (B(n,k) at b[n*ldb + k]) or "KN" form (b[k*ldb + n]),
-// which covers the forward convolution (im2col(X) x W^T), the input gradient
-// (dZ x W -> col2im) and the weight gradient (dZ^T x im2col(X)).  Work-group tile 64 x 64,
-// K step 32, four waves each owning a 32 x 32 block computed with
-// v_mfma_f32_32x32x16_bf16, float accumulation.
+//   A in "MK" form (A(m,k) at a[m*lda + k]), "KM" form (a[k*lda + m]) or "MKconv" form
+//     (the 3x3 neighbourhood of pixel m, k = tap*C + c, gathered from a channels-last image),
+//   B in "NK" form (B(n,k) at b[n*ldb + k]), "KN" form (b[k*ldb + n]) or "KNconv" form
+//     (n = tap*C + c, k = pixel),
+// which covers the forward convolution (MKconv(X) x W^T), the input gradient (MKconv(dZ) x
+// Wt^T with the reflected, transposed kernel Wt) and the weight gradient (dZ^T x KNconv(X)),
+// with no im2col buffer.  Work-group tile 64 x 64, K step 32, four waves each owning a 32 x 32
+// block computed with v_mfma_f32_32x32x16_bf16, float accumulation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,9 +31,16 @@ __device__ inline u16 bf16_bits(float v) {
 }
 __device__ inline float bf16_float(u16 b) { return __uint_as_float(((uint32_t)b) << 16); }
 
-enum : int { kMK = 0, kKM = 1 };             // A forms
-enum : int { kNK = 0, kKN = 1 };             // B forms
+enum : int { kMK = 0, kKM = 1, kMKconv = 2 };   // A forms
+enum : int { kNK = 0, kKN = 1, kKNconv = 2 };   // B forms
 enum : int { kStoreF32 = 0, kStoreBF16 = 1, kAtomicF32 = 2 };
+
+// Geometry of an implicit 3x3 (padding 1) convolution operand: the element for pixel p and
+// column j = tap * C + c is X[b, x + tap/3 - 1, y + tap%3 - 1, c] (zero off the image), with
+// p = (b*nx + x)*ny + y and X channels-last with channel stride ld (the operand's lda/ldb).
+struct ConvGeom {
+    int nx, ny, C;
+};
 
 struct GemmArgs {
     const u16* a;
@@ -42,16 +52,56 @@ struct GemmArgs {
     int c_off;           // column offset into c (channel slice of a concatenation)
     int relu;            // apply max(0, .) in the epilogue
     int k_split;         // K range per z-slice (multiple of 32)
+    ConvGeom conv;       // for kMKconv / kKNconv operands
 };
+
+// address of the 8 channels [c, c+8) of pixel p's neighbour through tap, or null off the image
+__device__ inline const u16* conv_src(const u16* X, int ld, const ConvGeom& g, long long p, int j) {
+    const int tap = j / g.C, c = j - tap * g.C;
+    const int y = (int)(p % g.ny);
+    const long long bx = p / g.ny;
+    const int x = (int)(bx % g.nx);
+    const long long b = bx / g.nx;
+    const int xx = x + tap / 3 - 1, yy = y + tap % 3 - 1;
+    if (xx < 0 || xx >= g.nx || yy < 0 || yy >= g.ny) return nullptr;
+    return X + ((size_t)(b * g.nx + xx) * g.ny + yy) * ld + c;
+}
 
 constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8, kLdsRow = kTK + kPad;
 
 // Loads a 64 x 32 tile of an operand into LDS as [row][k] (k contiguous), zero-filled
 // outside the matrix.  kform: 0 -> element (row, k) at p[row*ld + k]; 1 -> at p[k*ld + row].
+// FORM 2 (kMKconv): row = pixel, k = tap*C + c gathered from the image (rows x K = P x 9C).
+// FORM 3 (kKNconv): row = tap*C + c, k = pixel (rows x K = 9C x P).  C % 8 == 0, so every
+// 8-element chunk lies in one tap; K and row counts of conv operands are multiples of 8.
 template <int FORM>
 __device__ inline void load_tile(u16* lds, const u16* __restrict__ p, int ld, int row0, int rows,
-                                 int k0, int kend) {
+                                 int k0, int kend, const ConvGeom& cg) {
     const int t = threadIdx.x;  // 256 threads, 8 elements each
+    if (FORM == 2) {
+        const int r = t >> 2, kc = (t & 3) * 8;
+        const int gr = row0 + r, gk = k0 + kc;
+        uint4 q = {0, 0, 0, 0};
+        if (gr < rows && gk < kend) {
+            const u16* src = conv_src(p, ld, cg, gr, gk);
+            if (src) q = *(const uint4*)src;
+        }
+        *(uint4*)(lds + r * kLdsRow + kc) = q;
+        return;
+    }
+    if (FORM == 3) {
+        const int k = t >> 3, rc = (t & 7) * 8;
+        const int gk = k0 + k, gr = row0 + rc;
+        uint4 q = {0, 0, 0, 0};
+        if (gk < kend && gr < rows) {
+            const u16* src = conv_src(p, ld, cg, gk, gr);
+            if (src) q = *(const uint4*)src;
+        }
+        const u16* v = (const u16*)&q;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) lds[(rc + e) * kLdsRow + k] = v[e];
+        return;
+    }
     if (FORM == 0) {
         const int r = t >> 2, kc = (t & 3) * 8;
         const int gr = row0 + r, gk = k0 + kc;
@@ -93,8 +143,8 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc = {0};
     for (int k0 = kb; k0 < ke; k0 += kTK) {
-        load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke);
-        load_tile<BF>(sb, g.b, g.ldb, n0, g.N, k0, ke);
+        load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv);
+        load_tile<BF == kKNconv ? 3 : BF>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {
@@ -122,50 +172,15 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     }
 }
 
-// im2col for a 3x3, padding-1 convolution: col[p][tap * C + c] = X[b, x+dx, y+dy, c] (0 off
-// the image), tap = (dx+1)*3 + (dy+1), p = (b*nx + x)*ny + y.  X has channel stride ldx.
-__global__ void im2col3x3(const u16* __restrict__ X, int ldx, int xoff, u16* __restrict__ col,
-                          int nb, int nx, int ny, int C) {
+// Wt[ci][t * cout + co] = W[co][(8 - t) * cin + ci]: the kernel of the input gradient of a
+// 3x3 convolution (its taps reflected and its channels transposed), bf16
+__global__ void flip_weights(const u16* __restrict__ W, int cout, int cin, u16* __restrict__ Wt) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)nb * nx * ny * 9 * (C / 8);
-    if (t >= total) return;
-    const int c8 = (int)(t % (C / 8));
-    long long r = t / (C / 8);
-    const int tap = (int)(r % 9);
-    const long long p = r / 9;
-    const int y = (int)(p % ny);
-    const long long bx = p / ny;
-    const int x = (int)(bx % nx);
-    const int b = (int)(bx / nx);
-    const int xx = x + tap / 3 - 1, yy = y + tap % 3 - 1;
-    uint4 v = {0, 0, 0, 0};
-    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny)
-        v = *(const uint4*)(X + ((size_t)((size_t)b * nx + xx) * ny + yy) * ldx + xoff + c8 * 8);
-    *(uint4*)(col + (size_t)p * 9 * C + tap * C + c8 * 8) = v;
-}
-
-// col2im (gather): dX[p][c] (+)= sum_tap dcol[p - shift(tap)][tap * C + c]
-__global__ void col2im3x3(const float* __restrict__ dcol, float* __restrict__ dX, int lddx,
-                          int dxoff, int accumulate, int nb, int nx, int ny, int C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)nb * nx * ny * C;
-    if (t >= total) return;
-    const int c = (int)(t % C);
-    const long long p = t / C;
-    const int y = (int)(p % ny);
-    const long long bx = p / ny;
-    const int x = (int)(bx % nx);
-    const int b = (int)(bx / nx);
-    float s = 0.f;
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        // the output pixel (x - dx, y - dy) read this pixel through tap (dx, dy)
-        const int ox = x - (tap / 3 - 1), oy = y - (tap % 3 - 1);
-        if (ox >= 0 && ox < nx && oy >= 0 && oy < ny)
-            s += dcol[((size_t)((size_t)b * nx + ox) * ny + oy) * 9 * C + tap * C + c];
-    }
-    float* o = dX + (size_t)p * lddx + dxoff + c;
-    *o = accumulate ? *o + s : s;
+    if (t >= (long long)cout * 9 * cin) return;
+    const int ci = (int)(t % cin);
+    const int tap = (int)((t / cin) % 9);
+    const int co = (int)(t / (9 * cin));
+    Wt[((size_t)ci * 9 + (8 - tap)) * cout + co] = W[t];
 }
 
 // dZ = dY * (Y > 0) as bf16, dY float with channel stride ld / offset off
