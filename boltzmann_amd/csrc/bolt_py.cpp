@@ -695,6 +695,10 @@ PYBIND11_MODULE(_boltzmann, m) {
         .def("set_shape", [](PyBatch& b, int block, int graph_steps) {
             check(bolt_batch_set_shape(b.core->batch, block, graph_steps), "set_shape");
         }, py::arg("block"), py::arg("graph_steps"))
+        .def("set_active", [](PyBatch& b, py::array_t<bool, py::array::c_style | py::array::forcecast> a) {
+            std::vector<uint8_t> v(a.data(), a.data() + a.size());
+            check(bolt_batch_set_active(b.core->batch, v.data(), (int)v.size()), "set_active");
+        }, py::arg("active"))
         .def("tune", [](PyBatch& b, int steps) {
             char rep[2048];
             check(bolt_batch_tune(b.core->batch, steps, rep, sizeof rep), "tune");

@@ -1262,6 +1262,7 @@ struct bolt_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t cells = 0;
     bool tracers = true;
+    std::vector<uint8_t> active;   // instances a step advances (empty: all)
 };
 
 static void batch_drop_graph(bolt_batch* B) {
@@ -1273,18 +1274,22 @@ static void batch_drop_graph(bolt_batch* B) {
 static int batch_build_tiles(bolt_batch* B, int block) {
     std::vector<Tile> tiles;
     for (size_t k = 0; k < B->lat.size(); ++k) {
+        if (!B->active.empty() && !B->active[k]) continue;
         const bolt_lattice* L = B->lat[k];
         const int cpr = (L->n[1] + block - 1) / block;
         for (int r = 0; r < L->n[0]; ++r)
             for (int c = 0; c < cpr; ++c) tiles.push_back({(uint16_t)k, (uint16_t)r, (uint16_t)c, 0});
     }
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream));  // no launch still reads the old table
     hipFree(B->d_tiles);
     B->d_tiles = nullptr;
-    HIP_TRY(hipMalloc(&B->d_tiles, tiles.size() * sizeof(Tile)));
-    // on the batch's (non-blocking) stream, so the launches that read the table follow it
-    HIP_TRY(hipMemcpyAsync(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                           B->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    if (!tiles.empty()) {
+        HIP_TRY(hipMalloc(&B->d_tiles, tiles.size() * sizeof(Tile)));
+        // on the batch's (non-blocking) stream, so the launches that read the table follow it
+        HIP_TRY(hipMemcpyAsync(B->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                               B->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(B->ctx->stream));
+    }
     B->ntiles = (int)tiles.size();
     B->block = block;
     batch_drop_graph(B);
@@ -1322,6 +1327,7 @@ static int batch_capture(bolt_batch* B, int len) {
 
 static int batch_enqueue(bolt_batch* B, uint64_t n) {
     if (n % 2 != 0) return fail(BOLT_EINVAL, "iters must be even");
+    if (B->ntiles == 0) return BOLT_OK;  // every instance is inactive
     HIP_TRY(hipSetDevice(B->ctx->device));
     hipStream_t s = B->ctx->stream;
     const uint64_t len = (uint64_t)B->graph_steps;
@@ -1469,6 +1475,14 @@ int bolt_batch_equilibrate(bolt_batch* B) {
 int bolt_batch_step(bolt_batch* B, uint64_t n) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     return batch_enqueue(B, n);
+}
+
+int bolt_batch_set_active(bolt_batch* B, const uint8_t* active, int n) {
+    if (!B || !active) return fail(BOLT_EINVAL, "null argument");
+    if (n != (int)B->lat.size()) return fail(BOLT_EINVAL, "expected %zu flags, got %d", B->lat.size(), n);
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    B->active.assign(active, active + n);
+    return batch_build_tiles(B, B->block);
 }
 
 int bolt_batch_step_timed(bolt_batch* B, uint64_t n, double* ms) {

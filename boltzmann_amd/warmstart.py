@@ -185,23 +185,28 @@ class SolveResult:
     first_residual: np.ndarray
     history: np.ndarray = None   # [checks, cases] residual at every check
     check_every: int = 0
-    steps_run: int = 0           # steps the batch ran (all cases step together)
+    steps_run: int = 0           # steps the batch ran
+    case_steps: int = 0          # steps summed over the cases while they were in the batch
 
     @property
     def seconds_per_case_step(self) -> float:
-        """Wall time of one step of one case in this batch, residual checks included."""
-        return self.seconds / max(1, self.steps_run * len(self.steps))
+        """Wall time of one step of one case in this batch, set-up and residual checks
+        included."""
+        return self.seconds / max(1, self.case_steps)
 
 
 def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
-          batch: Batch | None = None, stop_when_done=True, keep_fields=True, neq=False):
+          batch: Batch | None = None, stop_when_done=True, keep_fields=True, neq=False,
+          drop_stalled: int = 0):
     """Runs every case to the residual tolerance in one batch.
 
     init: None for the cold start, else a list of (rho, vel) per case; with neq the
     initial populations carry the non-equilibrium part of populations(), else they are at
     equilibrium.  SolveResult.seconds covers the set-up (state upload, initial
-    populations) and the stepping.  Returns the SolveResult and, with keep_fields, the
-    final (rho, vel) of every case."""
+    populations) and the stepping.  With stop_when_done a converged case leaves the batch
+    (its state is kept); with drop_stalled > 0 so does a case whose residual has not halved
+    over that many checks (an unsteady flow; its steps stay -1).  Returns the SolveResult
+    and, with keep_fields, the final (rho, vel) of every case."""
     t0 = time.perf_counter()
     b = batch or _batch(cases, dev)
     written = []
@@ -220,24 +225,39 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
     first = np.full(n, np.nan)
     hist = []
     done = 0
-    while done < max_steps:
+    active = np.ones(n, bool)
+    best = np.full(n, np.inf)          # lowest residual so far, and the check it was halved at
+    best_at = np.zeros(n, np.int64)
+    checks = 0
+    case_steps = 0
+    while done < max_steps and active.any():
+        case_steps += int(active.sum()) * check_every
         b.step(check_every - 100)
-        for k in range(n):
+        for k in np.flatnonzero(active):
             b.residual(k, 0)          # snapshot u^(n-100)
         b.step(100)
         done += check_every
-        for k in range(n):
+        checks += 1
+        for k in np.flatnonzero(active):
             r = b.residual(k, 100)
             last[k] = r
             if np.isnan(first[k]):
                 first[k] = r
             if steps[k] < 0 and r <= tol:
                 steps[k] = done
+            if r < 0.5 * best[k]:
+                best[k], best_at[k] = r, checks
         hist.append(last.copy())
-        if stop_when_done and (steps >= 0).all():
-            break
+        leave = np.zeros(n, bool)
+        if stop_when_done:
+            leave |= steps >= 0
+        if drop_stalled > 0:
+            leave |= (steps < 0) & (checks - best_at >= drop_stalled)
+        if (active & leave).any():
+            active &= ~leave
+            b.set_active(active)
     secs = time.perf_counter() - t0
-    res = SolveResult(steps, last, secs, first, np.array(hist), check_every, done)
+    res = SolveResult(steps, last, secs, first, np.array(hist), check_every, done, case_steps)
     if not keep_fields:
         return res, None
     fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
@@ -275,13 +295,17 @@ def mirrored_features(x: np.ndarray) -> np.ndarray:
 
 
 def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
-             check_every: int = 2000, tol: float = 1e-6, log=None, **case_kw):
+             check_every: int = 2000, tol: float = 1e-6, log=None, drop_stalled: int = 8,
+             **case_kw):
     """Solves n cases (seeds seed0 ..) to the tolerance in batches; returns the converged
-    cases, their steady (rho, vel) and the steps they took."""
+    cases, their steady (rho, vel) and the steps they took.  Converged cases leave their
+    batch, and so do cases whose residual stalls for drop_stalled checks (unsteady flows,
+    which would be dropped anyway)."""
     cases, fields, steps = [], [], []
     for b0 in range(0, n, batch):
         cs = [make_case(seed0 + s, **case_kw) for s in range(b0, min(n, b0 + batch))]
-        res, fl = solve(cs, tol=tol, check_every=check_every, max_steps=max_steps)
+        res, fl = solve(cs, tol=tol, check_every=check_every, max_steps=max_steps,
+                        drop_stalled=drop_stalled)
         for c, f, st in zip(cs, fl, res.steps):
             if st >= 0 and np.isfinite(f[1]).all():
                 cases.append(c)

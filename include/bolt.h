@@ -196,6 +196,11 @@ int bolt_batch_cells(const bolt_batch* b, uint64_t* out);
 int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
+/* instances that later steps advance (active[n], non-zero: advance); an inactive instance
+ * keeps its state and can still be read, reduced or checked for its residual.  Change it
+ * only between step calls (each of an even count, so every instance stays in its natural
+ * population layout). */
+int bolt_batch_set_active(bolt_batch* b, const uint8_t* active, int n);
 /* autotuner: times the candidate shapes on this batch (state restored afterwards), keeps
  * the fastest and writes a per-candidate report into report[len]. */
 int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);

@@ -119,3 +119,24 @@ def test_rb_sweep_small_runs_and_conserves():
         assert np.isfinite(T).all()
         np.testing.assert_array_equal(T[:, 1], 1.0)
         np.testing.assert_array_equal(T[:, -2], 0.0)
+
+
+def test_inactive_instances_keep_their_state_and_the_others_run_on():
+    coll = "trt"
+    sp = specs(coll)
+    b = build_batch(sp, coll)
+    b.step(10)
+    frozen = b.read(1, "f").copy()
+    active = np.ones(len(sp), bool)
+    active[1] = False
+    b.set_active(active)
+    b.step(14)
+    np.testing.assert_array_equal(b.read(1, "f"), frozen)            # not advanced
+    for k in (0, 2):
+        np.testing.assert_array_equal(b.read(k, "f"), oracle_run(sp[k], coll, 24).f)
+    b.set_active(np.zeros(len(sp), bool))
+    b.step(4)                                                        # nothing to advance
+    np.testing.assert_array_equal(b.read(1, "f"), frozen)
+    b.set_active(np.ones(len(sp), bool))
+    b.step(14)                                                       # instance 1 resumes
+    np.testing.assert_array_equal(b.read(1, "f"), oracle_run(sp[1], coll, 24).f)
