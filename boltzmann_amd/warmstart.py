@@ -394,7 +394,7 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
         spent = time.perf_counter() - t0
         if e0 == 1 and time_budget is not None:
             epochs = max(1, min(epochs, int(time_budget / spent)))
-        if log and (e0 % log_every == 0 or e0 == epochs):
+        if log:
             log(f"epoch {e0}/{epochs}: loss {losses[-1]:.5f} ({spent:.1f} s)")
     return model, np.array(losses, np.float32)
 
