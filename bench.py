@@ -249,6 +249,28 @@ def memory_ceilings():
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
+def nn_throughput(nx: int = 512, ny: int = 256, batch: int = 8, width: int = 32, depth: int = 6,
+                  samples: int = 64, epochs: int = 2):
+    """Training and inference throughput of the warm-start network on random data of the
+    512 x 256 channel size (the GEMM kernels' workload), after one warm-up epoch."""
+    import boltzmann_amd
+    net = boltzmann_amd.Network("gpu", nx, ny, batch, width=width, seed=0, depth=depth)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((samples, nx, ny, 8)).astype(np.float32)
+    y = rng.standard_normal((samples, nx, ny, 3)).astype(np.float32)
+    net.set_data(x, y)
+    net.train(1, lr=1e-3)
+    t0 = time.perf_counter()
+    net.train(epochs, lr=1e-3)
+    train = epochs * samples / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    net.predict(x[:32])
+    infer = 32 / (time.perf_counter() - t0)
+    return {"lattice": [nx, ny], "depth": depth, "width": width, "batch": batch,
+            "params": int(net.param_count), "train_samples_per_s": round(train, 1),
+            "predict_samples_per_s": round(infer, 1)}
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of nbytes (hipMemcpy): a kernel of known traffic for
     scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -285,6 +307,8 @@ def main() -> int:
     ap.add_argument("--ws-train", type=int, default=2000)
     ap.add_argument("--ws-test", type=int, default=100)
     ap.add_argument("--ws-epochs", type=int, default=20)
+    ap.add_argument("--nn-throughput", action="store_true",
+                    help="add the warm-start network's training / inference throughput")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -324,6 +348,9 @@ def main() -> int:
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
+    if args.nn_throughput:
+        variants["nn_unet_512x256"] = nn_throughput()
+        print(f"[bench] network: {variants['nn_unet_512x256']}", file=sys.stderr, flush=True)
     if args.warmstart:
         variants["warmstart_channel_512x256"] = warmstart_channel(
             args.ws_train, args.ws_test, args.ws_epochs)
