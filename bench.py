@@ -96,6 +96,25 @@ def run_gpu(n, coll, storage, steps, warmup, rho, vel, keep_vel=False):
     return out
 
 
+def run_sim_style(n, rho, vel, calls: int = 5, per_call: int = 100):
+    """The reference's run_sim rate (python/boltzmann/simulation.py:235-246): wall time of
+    iterate(per_call), which reads rho and u back to the host, plus the non-finite check."""
+    import boltzmann_amd
+    sim = boltzmann_amd.Simulation("gpu", [n, n, n], omega("trt"), storage="f32")
+    sim.fluid.rho[...] = rho
+    sim.fluid.vel[...] = vel
+    sim.iterate(2)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        sim.iterate(per_call)
+        if np.any(~np.isfinite(sim.fluid.vel)):
+            raise ValueError("Non-finite value detected!")
+    secs = time.perf_counter() - t0
+    del sim
+    return {"mlups": round(n ** 3 * per_call * calls / (1e6 * secs), 1), "steps_per_call": per_call,
+            "calls": calls}
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """Oracle D3Q27 TRT on a 64^3 TGV sample, all host threads, about budget_s seconds."""
     import oracle
@@ -327,6 +346,9 @@ def main() -> int:
 
     head = run_gpu(n, "trt", "f32", steps, args.warmup, rho, vel)
     print(f"[bench] f32 trt: {head['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    run_sim_rate = run_sim_style(n, rho, vel)
+    print(f"[bench] run_sim-style (iterate + read-back): {run_sim_rate['mlups']:.0f} MLUPS",
+          file=sys.stderr, flush=True)
     variants = {}
     if not args.no_variants:
         ref64 = run_gpu(n, "trt", "f64", max(2, min(steps, 200)), 0, rho, vel, keep_vel=True)
@@ -378,6 +400,7 @@ def main() -> int:
                      "traffic": measured_traffic() if n == 256 else None,
                      "bytes_per_lup": B_ALG["f32"]},
         "wall_mlups": round(head["wall_mlups"], 1),
+        "run_sim_mlups": run_sim_rate,
         "memory_ceilings_gbs": ceilings,
         "frac_of_inplace_ceiling": (round(achieved / ceilings["inplace27"], 4)
                                     if ceilings and "inplace27" in ceilings else None),
