@@ -114,3 +114,15 @@ def test_host_equilibrium_matches_oracle_bit_exactly():
                                                    np.float32(np.float32(k45 * vq) * vq)) -
                                         np.float32(k15 * vv)))
             assert f[n, i] == val
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the measurement is one GPU's; the refusal happens before anything touches a device
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 2, res
+    assert "one GPU" in res.stderr or "single MI355X" in res.stderr, res.stderr
+    assert res.stdout.strip() == ""
