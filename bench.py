@@ -156,11 +156,18 @@ def rb_batch(steps: int = 50000):
         T = b.read(k, "C")
         uy = b.read(k, "vel")[..., 1]
         nus.append(round(rb.nusselt(T, uy, specs[k]), 3))
+    # physics sanity: aspect-2 cells run to developed convection (subcritical: Nu = 1)
+    check = [rb.spec(256, 130, ra, seed=2000 + k) for k, ra in enumerate((1000.0, 1e4, 1e5))]
+    cb = rb.make_batch(check)
+    cb.step(400_000)
+    nu_check = {f"{s.ra:.0e}": round(rb.nusselt(cb.read(k, "C"), cb.read(k, "vel")[..., 1], s), 4)
+                for k, s in enumerate(check)}
     return {"instances": len(specs), "cells": int(cells), "steps": steps,
             "mlups": round(mlups, 1), "ms_per_step": round(ms / steps, 5),
             "hbm_frac": round(mlups * 1e6 * 121 / (PEAK_HBM_GBS * 1e9), 4),
             "shape": report.strip().splitlines()[-1],
-            "nusselt_first4": nus, "ra_first4": [round(s.ra) for s in specs[:4]]}
+            "nusselt_first4": nus, "ra_first4": [round(s.ra) for s in specs[:4]],
+            "nusselt_256x130_400k_steps": nu_check}
 
 
 def cylinder(steps_marks=(1000, 5000, 20000)):

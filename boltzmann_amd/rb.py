@@ -51,11 +51,16 @@ def sweep(n: int = 64, seed: int = 4, u_ff: float = 0.05, pr: float = 1.0) -> li
         nx = int(rng.choice(nxs))
         ny = int(rng.choice(nys))
         ra = float(10 ** rng.uniform(4, 6))
-        h = ny - 2
-        nu = u_ff * h * np.sqrt(pr / ra)
-        gbeta = u_ff ** 2 / h
-        out.append(RBSpec(nx, ny, ra, 3 * nu + 0.5, -gbeta, 1000 + k))
+        out.append(spec(nx, ny, ra, 1000 + k, u_ff, pr))
     return out
+
+
+def spec(nx: int, ny: int, ra: float, seed: int = 0, u_ff: float = 0.05,
+         pr: float = 1.0) -> RBSpec:
+    """One cell of the sweep's parametrisation at a chosen size and Rayleigh number."""
+    h = ny - 2
+    nu = u_ff * h * np.sqrt(pr / ra)
+    return RBSpec(nx, ny, ra, 3 * nu + 0.5, -(u_ff ** 2 / h), seed)
 
 
 def initial_state(spec: RBSpec):

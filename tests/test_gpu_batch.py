@@ -140,3 +140,21 @@ def test_inactive_instances_keep_their_state_and_the_others_run_on():
     b.set_active(np.ones(len(sp), bool))
     b.step(14)                                                       # instance 1 resumes
     np.testing.assert_array_equal(b.read(1, "f"), oracle_run(sp[1], coll, 24).f)
+
+
+def test_rayleigh_benard_onset_and_nusselt():
+    # aspect-2 cells (one roll pair of wavelength 2H, near the critical wavenumber 3.12):
+    # below the rigid-wall critical Rayleigh number 1708 the noise decays and heat is only
+    # conducted (Nu = 1); above it convection sets in and carries heat, more at higher Ra
+    from boltzmann_amd import rb
+    specs = [rb.spec(256, 130, ra, seed=2000 + k) for k, ra in enumerate((1000.0, 1e4, 1e5))]
+    b = rb.make_batch(specs)
+    nus = []
+    for steps in (300_000, 100_000):
+        b.step(steps)
+        nus.append([rb.nusselt(b.read(k, "C"), b.read(k, "vel")[..., 1], s)
+                    for k, s in enumerate(specs)])
+    (sub, lo, hi), (sub2, lo2, hi2) = nus
+    assert abs(sub2 - 1.0) < 1e-3, nus
+    assert abs(lo2 / lo - 1) < 1e-2 and abs(hi2 / hi - 1) < 1e-2, nus   # steady rolls
+    assert 2.0 < lo2 < 3.5 and 4.0 < hi2 < 6.5 and hi2 > lo2, nus
