@@ -132,6 +132,41 @@ __device__ inline void load_tile(u16* lds, const u16* __restrict__ p, int ld, in
     }
 }
 
+// 8 consecutive rows [r, r+8) at one k of a k-major operand (FORM 1: p[k*ld + row];
+// FORM 3: the gathered image, k = pixel, row = tap*C + c), zero outside the matrix
+template <int FORM>
+__device__ inline uint4 load8_kmajor(const u16* __restrict__ p, int ld, int r, int rows, int k, int kend,
+                                     const ConvGeom& cg) {
+    uint4 q = {0, 0, 0, 0};
+    if (k >= kend || r >= rows) return q;
+    if (FORM == 3) {
+        const u16* src = conv_src(p, ld, cg, k, r);
+        if (src) q = *(const uint4*)src;
+        return q;
+    }
+    if (r + 8 <= rows && (ld & 7) == 0) return *(const uint4*)(p + (size_t)k * ld + r);
+    u16* v = (u16*)&q;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (r + e < rows) ? p[(size_t)k * ld + r + e] : (u16)0;
+    return q;
+}
+
+// Fills one 64 x 32 LDS tile ([row][k]) of a k-major operand with 128 threads: thread t
+// loads k = 2*(t>>3) and k+1 for rows 8*(t&7) .. +7 and writes the 8 (k, k+1) pairs as
+// 4-byte stores (half the LDS store instructions of one k per thread)
+template <int FORM>
+__device__ inline void load_tile_kpairs(u16* lds, const u16* __restrict__ p, int ld, int row0, int rows,
+                                        int k0, int kend, const ConvGeom& cg, int t) {
+    const int kp = 2 * (t >> 3), rc = (t & 7) * 8;
+    const uint4 q0 = load8_kmajor<FORM>(p, ld, row0 + rc, rows, k0 + kp, kend, cg);
+    const uint4 q1 = load8_kmajor<FORM>(p, ld, row0 + rc, rows, k0 + kp + 1, kend, cg);
+    const u16* v0 = (const u16*)&q0;
+    const u16* v1 = (const u16*)&q1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        *(uint32_t*)(lds + (rc + e) * kLdsRow + kp) = (uint32_t)v0[e] | ((uint32_t)v1[e] << 16);
+}
+
 template <int AF, int BF, int OUT>
 __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     __shared__ __attribute__((aligned(16))) u16 sa[kTM * kLdsRow];
@@ -143,8 +178,17 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc = {0};
     for (int k0 = kb; k0 < ke; k0 += kTK) {
-        load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv);
-        load_tile<BF == kKNconv ? 3 : BF>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv);
+        if constexpr (AF == kKM && (BF == kKN || BF == kKNconv)) {
+            // both operands k-major (the weight gradient): half the threads fill each tile
+            if (threadIdx.x < 128)
+                load_tile_kpairs<1>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv, threadIdx.x);
+            else
+                load_tile_kpairs<BF == kKNconv ? 3 : 1>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv,
+                                                        threadIdx.x - 128);
+        } else {
+            load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv);
+            load_tile<BF == kKNconv ? 3 : BF>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv);
+        }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {

@@ -365,3 +365,16 @@ def test_mass_conservation_large_d3q27():
     m1 = sim.reduce("mass")
     assert abs(m1 - m0) / m0 < 1e-6
     assert np.isfinite(sim.fluid.vel).all()
+
+
+def test_size_limits_are_refused_before_allocating():
+    # every population plane is addressed with 32-bit byte offsets: at most 2^32 bytes
+    bolt, core = _bolt()
+    with pytest.raises(bolt.BoltError, match="2\\^32 bytes per population plane"):
+        bolt.Simulation("gpu", [1100, 1100, 1100], core.bgk(0.6))          # 5.3 GB planes
+    with pytest.raises(bolt.BoltError, match="2\\^32 bytes per population plane"):
+        bolt.Simulation("gpu", [820, 820, 820], core.bgk(0.6), storage="f64")
+    with pytest.raises(bolt.BoltError, match="out of range"):
+        bolt.Simulation("gpu", [0, 16], core.bgk(0.6))
+    sim = bolt.Simulation("gpu", [800, 800, 8], core.bgk(0.6), storage="f64")  # within
+    assert sim.device_bytes > 0
