@@ -178,8 +178,11 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc = {0};
     for (int k0 = kb; k0 < ke; k0 += kTK) {
-        if constexpr (AF == kKM && (BF == kKN || BF == kKNconv)) {
-            // both operands k-major (the weight gradient): half the threads fill each tile
+        bool pairs = false;
+        if constexpr (AF == kKM && (BF == kKN || BF == kKNconv)) pairs = g.M >= 16;
+        if (pairs) {
+            // both operands k-major (the weight gradients of the 3x3 layers): half the threads
+            // fill each tile; a skinny A (the 1x1 output layer, M = 3) keeps one k per thread
             if (threadIdx.x < 128)
                 load_tile_kpairs<1>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv, threadIdx.x);
             else
