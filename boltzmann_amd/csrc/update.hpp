@@ -254,29 +254,10 @@ __device__ inline void ref_bgk(float (&f)[V::q], const float (&u)[3], float vv, 
     }
 }
 
-template <class V, int I = 0>
-__device__ inline void ref_trt(float (&f)[V::q], const float (&u)[3], float vv, float r,
-                               float omp, float omn) {
-#pragma clang fp contract(off)
-    if constexpr (I < V::q) {
-        constexpr int J = V::opp(I);
-        if constexpr (J >= I) {
-            const double pp = ref_feq_pos<V, I>(u, vv);
-            const double pn = ref_feq_neg<V, I>(u);
-            const float fpos =
-                (float)((double)omp * ((double)r * pp - (double)(f[I] + f[J]) * 0.5));
-            const float fneg =
-                (float)((double)omn * ((double)r * pn - (double)(f[I] - f[J]) * 0.5));
-            f[I] = f[I] + (fpos + fneg);
-            if constexpr (J != I) f[J] = f[J] + (fpos - fneg);
-        }
-        ref_trt<V, I + 1>(f, u, vv, r, omp, omn);
-    }
-}
-
-// Collision fused with the AA stores, one opposite pair (I, opp I) at a time, so each
-// population's register is free as soon as it is stored.  Same arithmetic, same results
-// as ref_bgk / ref_trt followed by aa_store_raw.
+// The reference-arithmetic collision (lib.cl:266-274 BGK, 701-719 TRT: f+/- halves,
+// omega+ on the even part, omega- on the odd part, double polynomials narrowed to float)
+// fused with the AA stores, one opposite pair (I, opp I) at a time, so each population's
+// register is free as soon as it is stored.
 template <class V, bool EVEN, int COLL, class NB, int I = 0>
 __device__ inline void ref_collide_store(float (&f)[V::q], const float (&u)[3], float vv, float r,
                                          float omp, float omn, void* const* planes, const NB& nb,
@@ -634,13 +615,7 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
         }
         if constexpr (S == kF32) {
             const float vv = ref_vv<D>(u);
-#ifdef BOLT_SEPARATE_STORES
-            if constexpr (COLL == 0) ref_bgk<V>(f, u, vv, r, omp);
-            else ref_trt<V>(f, u, vv, r, omp, omn);
-            aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
-#else
             ref_collide_store<V, EVEN, COLL>(f, u, vv, r, omp, omn, p.fpl, nb, cell);
-#endif
         } else if constexpr (S == kF32F) {
             const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
             if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
