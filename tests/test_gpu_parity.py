@@ -55,7 +55,8 @@ def mixed_flags(counts, rng, walls=0.12, fixed=0.05, scalar=0.0):
 
 
 def make_pair(counts, om, q=None, storage="f32", seed=0, walls=0.12, fixed=0.05,
-              gravity=None, tracers=(), couplings=(), scalar=0.0, vel_scale=0.04):
+              gravity=None, tracers=(), couplings=(), scalar=0.0, vel_scale=0.04,
+              dtype=np.float32):
     """Builds the product Simulation and the oracle with identical inputs."""
     bolt, core = _bolt()
     rng = np.random.default_rng(seed)
@@ -64,7 +65,7 @@ def make_pair(counts, om, q=None, storage="f32", seed=0, walls=0.12, fixed=0.05,
     rho = (1 + 0.02 * rng.standard_normal(counts)).astype(np.float32)
     vel = (vel_scale * rng.standard_normal(counts + (d,))).astype(np.float32)
     sim = bolt.Simulation("gpu", list(counts), omega_obj(om), q, storage=storage)
-    ref = oracle.OracleSimulation(counts, om, q=q)
+    ref = oracle.OracleSimulation(counts, om, q=q, dtype=dtype)
     sim.cells.flags[...] = flags
     sim.fluid.rho[...] = rho
     sim.fluid.vel[...] = vel
@@ -315,6 +316,61 @@ def test_f64_storage_matches_oracle_f64(counts, q, om):
     sim.iterate(6)
     ref.iterate(6)
     np.testing.assert_allclose(sim.read_field("f", dtype="f64"), ref.f, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("counts,q,om", SETS)
+def test_f64_tracers_and_coupling_match_oracle_f64(counts, q, om):
+    # alpha and c0 exact in float32, so the kernels' float parameters equal the oracle's
+    d = len(counts)
+    g = [0.0, -0.002, 0.001][:d]
+    sim, ref, flags = make_pair(counts, om, q, storage="f64", seed=8, gravity=g,
+                                tracers=[("T", f32(1 / 0.7)), ("S", f32(1 / 0.9))],
+                                couplings=[("T", 0.0625, 0.5, g)], scalar=0.05,
+                                dtype=np.float64)
+    sim.iterate(6)
+    ref.iterate(6)
+    np.testing.assert_allclose(sim.read_field("f", dtype="f64"), ref.f, rtol=0, atol=1e-13)
+    for name, t in ref.tracers.items():
+        np.testing.assert_allclose(sim.read_field("g", name, dtype="f64"), t.g, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("q", [9, 27])
+def test_reduced_precision_with_tracer_and_coupling(q):
+    # every storage format against the f64 run of the same kernels, with the separate
+    # (non-fused) tracer kernel that f16s / bf16s / f32f use
+    bolt, core = _bolt()
+    counts = (48, 40) if q == 9 else (12, 14, 24)
+    d = len(counts)
+    g = [0.0, -1e-3, 0.0][:d]
+    rng = np.random.default_rng(21)
+    flags = np.zeros(counts, np.int32)
+    flags[:, 0] = flags[:, -1] = 1
+    flags[:, 1] |= 8
+    C = rng.random(counts).astype(np.float32)
+    C[:, 1] = 1.0
+    vel = (0.02 * rng.standard_normal(counts + (d,))).astype(np.float32)
+    vel[flags == 1] = 0
+    out = {}
+    for st in ("f64", "f32", "f32f", "f16s", "bf16s"):
+        om = core.Omega.TRT(1 / 0.7, 1 / (0.25 / 0.2 + 0.5))
+        sim = bolt.Simulation("gpu", list(counts), om, storage=st)
+        sim.cells.flags[...] = flags
+        sim.fluid.vel[...] = vel
+        tr = sim.add_tracer("T", core.Omega.BGK(1 / 0.8))
+        tr.val[...] = C
+        sim.add_boussinesq_coupling(tr, 0.0625, 0.5, np.asarray(g, np.float32))
+        sim.iterate(200)
+        out[st] = (sim.read_field("vel", dtype="f64"), sim.read_field("C", "T", dtype="f64"))
+    u64, c64 = out["f64"]
+    umax = np.abs(u64).max()
+    err = {st: (np.abs(u - u64).max() / umax, np.abs(c - c64).max()) for st, (u, c) in out.items()}
+    print("velocity / tracer max errors against f64:", err, flush=True)
+    # measured (D2Q9 / D3Q27): f32 2.6e-5 / 1.8e-4 -- float rounding of populations near w_i
+    # on a velocity field that decays to a few 1e-3 --, f32f 6.7e-6 / 3.3e-5, f16s 4.2e-4 /
+    # 6.0e-4, bf16s 3.1e-3 / 2.9e-3; tracer errors 2e-7 .. 1.1e-5
+    bound = {"f32": 1e-3, "f32f": 2e-4, "f16s": 3e-3, "bf16s": 2e-2}
+    for st, b in bound.items():
+        assert err[st][0] < b and err[st][1] < 1e-4, (st, err[st])
 
 
 def tgv_inputs(n, u0=0.04):
