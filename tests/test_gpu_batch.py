@@ -158,3 +158,24 @@ def test_rayleigh_benard_onset_and_nusselt():
     assert abs(sub2 - 1.0) < 1e-3, nus
     assert abs(lo2 / lo - 1) < 1e-2 and abs(hi2 / hi - 1) < 1e-2, nus   # steady rolls
     assert 2.0 < lo2 < 3.5 and 4.0 < hi2 < 6.5 and hi2 > lo2, nus
+
+
+def test_fluid_only_batch_matches_oracle_bit_exactly():
+    # the warm start's batches: D2Q9 TRT channels with walls, obstacles and fixed in/outflow
+    # columns, no tracer (the batch kernel's fluid-only instantiation)
+    from boltzmann_amd import warmstart as ws
+    cases = [ws.make_case(77 + s, nx=64, ny=48, size_range=(4, 8)) for s in range(3)]
+    b = ws._batch(cases)
+    for k, c in enumerate(cases):
+        ws.load_state(b, k, c, *c.cold_state())
+    b.equilibrate()
+    b.step(30)
+    for k, c in enumerate(cases):
+        wp, wn = (float(np.float32(w)) for w in c.omega)
+        ref = oracle.OracleSimulation((c.nx, c.ny), ("trt", wp, wn))
+        ref.flags[...] = c.flags()
+        ref.rho[...], ref.vel[...] = c.cold_state()
+        ref.iterate(30)
+        np.testing.assert_array_equal(b.read(k, "f"), ref.f)
+        nw = ~c.mask
+        np.testing.assert_array_equal(b.read(k, "vel")[nw], ref.vel[nw])
