@@ -39,10 +39,10 @@ def test_warm_start_needs_fewer_steps_than_cold_start():
     model, losses = ws.train(cases, fields, epochs=60, batch=8, width=32, depth=5, lr=2e-3, seed=0)
     assert np.isfinite(losses).all() and losses[-1] < 0.05 * losses[0], losses
     held_out = [ws.make_case(50_000 + s, **KW) for s in range(32)]
-    cmp = ws.compare(model, held_out, check_every=200, max_steps=40000)
+    cmp = ws.compare(model, held_out, check_every=200, max_steps=40000, mirror=True)
     s = cmp.summary()
     print("warm start", s, flush=True)
     assert s["cases"] >= 24
-    # measured 0.78 (and 0.72 with ten times the training data); a network that predicted
+    # measured about 0.78 (0.68 with ten times the training data); a network that predicted
     # nothing useful scores about 1.0, and a rejected prediction costs judge_at extra steps
     assert s["median_step_ratio"] < 0.85, s
