@@ -15,7 +15,10 @@ the oracle (OpenMP C++ restatement of the reference's kernels) on a bounded samp
 roofline.achieved = 217 B per lattice update (27 float populations read and written
 once, 1 flag byte) * cells / average launch duration.  memory_ceilings_gbs reports what
 plain copy / read / write / in-place plane kernels moving the same bytes reach on the same
-GPU (boltzmann_amd/bw_probe); inplace27 is the ceiling for the update's access mix.
+GPU (boltzmann_amd/bw_probe).  The AA update alternates two access mixes: one step reads
+and writes each cell's own slots (the probe's inplace27), the next reads and writes the
+neighbours' slots (shift27).  frac_of_aa_pair_ceiling divides the achieved rate by their
+harmonic mean, the rate of a step pair that ran each mix at its probe ceiling.
 """
 
 from __future__ import annotations
@@ -411,6 +414,9 @@ def main() -> int:
         "memory_ceilings_gbs": ceilings,
         "frac_of_inplace_ceiling": (round(achieved / ceilings["inplace27"], 4)
                                     if ceilings and "inplace27" in ceilings else None),
+        "frac_of_aa_pair_ceiling": (
+            round(achieved * 0.5 * (1 / ceilings["inplace27"] + 1 / ceilings["shift27"]), 4)
+            if ceilings and "shift27" in ceilings else None),
         "mass_drift": head["mass_drift"],
         "variants": variants,
     }

@@ -6,7 +6,10 @@
 //   read27        27 float planes read, nothing written
 //   write27       27 float planes written, nothing read
 //   inplace27     27 float planes read and written back in place (the update's mix)
-//   inplace27_h   the same with 2-byte elements (the f16s/bf16s storage)
+//   shift27       the same, each population at its D3Q27 neighbour (periodic): the
+//                 access mix of the step that streams, where 18 of the 27 accesses of a
+//                 wavefront start one element off a 256-byte boundary
+//   inplace27_h   the same as inplace27 with 2-byte elements (the f16s/bf16s storage)
 //   inplace9      9 float planes in place over 3x the cells (the D2Q9 mix)
 // (each in-place figure the better of plain and streaming loads) and prints one JSON
 // object of GB/s (bytes read + written per second).  Every index is
@@ -64,6 +67,25 @@ __global__ void inplace_planes(T* __restrict__ f, long long n) {
     for (int q = 0; q < Q; ++q) f[q * n + i] = v[(q + 1) % Q];
 }
 
+// Each element (q, j) is read and written by exactly one thread: for a fixed q the map
+// i -> (i + shift_q) mod n is a bijection.  n is a power of two, so the wrap is a mask.
+template <bool NT>
+__global__ void shifted_planes(float* __restrict__ f, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[27];
+    long long a[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const long long d = (q % 3 - 1) + 256 * ((q / 3) % 3 - 1) + 65536 * (q / 9 - 1);
+        a[q] = q * n + ((i + d) & (n - 1));
+        if constexpr (NT) v[q] = __builtin_nontemporal_load(f + a[q]);
+        else v[q] = f[a[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[a[q]] = v[(q + 1) % 27];
+}
+
 template <typename F>
 static double gbs(F launch, double bytes, int reps) {
     hipEvent_t a, b;
@@ -106,6 +128,8 @@ int main(int argc, char** argv) {
     auto best = [](double a, double c) { return a > c ? a : c; };
     double r_in = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n); }, lat, reps),
                        gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n); }, lat, reps));
+    double r_sh = best(gbs([&] { shifted_planes<false><<<grid(n), b>>>(f, n); }, lat, reps),
+                       gbs([&] { shifted_planes<true><<<grid(n), b>>>(f, n); }, lat, reps));
     // 2-byte elements: 27 planes of n halves occupy half of the first lattice
     const double half = 2.0 * 2.0 * (double)total;
     double r_inh = best(
@@ -119,8 +143,9 @@ int main(int argc, char** argv) {
     hipDeviceProp_t p;
     CK(hipGetDeviceProperties(&p, 0));
     printf("{\"device\": \"%s\", \"bytes_per_pass\": %.0f, \"copy4\": %.1f, \"read27\": %.1f, "
-           "\"write27\": %.1f, \"inplace27\": %.1f, \"inplace27_h\": %.1f, \"inplace9\": %.1f}\n",
-           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_inh, r_in9);
+           "\"write27\": %.1f, \"inplace27\": %.1f, \"shift27\": %.1f, \"inplace27_h\": %.1f, "
+           "\"inplace9\": %.1f}\n",
+           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_in9);
     CK(hipFree(f));
     CK(hipFree(sink));
     return 0;
