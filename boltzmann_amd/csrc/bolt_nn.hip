@@ -104,7 +104,7 @@ static int nn_alloc(bolt_nn* N, void** p, size_t bytes) {
 // ---- launch helpers ---------------------------------------------------------------------
 static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, const u16* b, int ldb,
                  void* c, int ldc, int c_off, int M, int Nn, int K, const float* bias, bool relu,
-                 int k_split = 0, ConvGeom cg = {1, 1, 8}) {
+                 int k_split = 0, ConvGeom cg = conv_geom(1, 1, 8)) {
     GemmArgs g;
     g.conv = cg;
     g.a = a;
@@ -153,7 +153,7 @@ static void conv_fwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, u1
     const long long P = (long long)N->batch * nx * ny;
     if (L.taps == 9)
         gemm(N, kMKconv, kNK, kStoreBF16, X, L.cin, N->Wb + L.w_off, 9 * L.cin, Y, ldy, yoff, (int)P, L.cout,
-             9 * L.cin, N->W + L.b_off, L.relu, 0, ConvGeom{nx, ny, L.cin});
+             9 * L.cin, N->W + L.b_off, L.relu, 0, conv_geom(nx, ny, L.cin));
 }
 
 // backward of a 3x3 conv whose output was written at (Y, ldy, yoff) with gradient (dY, lddy,
@@ -167,14 +167,14 @@ static void conv_bwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, co
     bias_grad_launch(N, P, L.cout, N->G + L.b_off);
     // dW[co][tap*cin + ci] += sum_p dz[p][co] X[p + shift(tap)][ci]  (B gathers X)
     gemm(N, kKM, kKNconv, kAtomicF32, N->dz, L.cout, X, L.cin, N->G + L.w_off, 9 * L.cin, 0, L.cout,
-         9 * L.cin, (int)P, nullptr, false, 4096, ConvGeom{nx, ny, L.cin});
+         9 * L.cin, (int)P, nullptr, false, 4096, conv_geom(nx, ny, L.cin));
     if (dX) {
         // dX[p][ci] = sum_tap sum_co dz[p + shift(tap)][co] W[co][(8 - tap)*cin + ci]: the
         // forward form on dz with the reflected, transposed kernel Wt
         const long long nw = (long long)L.cout * 9 * L.cin;
         hipLaunchKernelGGL(flip_weights, dim3(nblk(nw)), dim3(256), 0, s, N->Wb + L.w_off, L.cout, L.cin, N->wt);
         gemm(N, kMKconv, kNK, kStoreF32, N->dz, L.cout, N->wt, 9 * L.cout, dX, L.cin, 0, (int)P, L.cin,
-             9 * L.cout, nullptr, false, 0, ConvGeom{nx, ny, L.cout});
+             9 * L.cout, nullptr, false, 0, conv_geom(nx, ny, L.cout));
     }
 }
 

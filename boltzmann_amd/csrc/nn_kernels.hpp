@@ -35,12 +35,37 @@ enum : int { kMK = 0, kKM = 1, kMKconv = 2 };   // A forms
 enum : int { kNK = 0, kKN = 1, kKNconv = 2 };   // B forms
 enum : int { kStoreF32 = 0, kStoreBF16 = 1, kAtomicF32 = 2 };
 
+// Unsigned 32-bit division by a divisor fixed for a launch, as a multiply-high (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI 1994): with
+// s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1, n / d = (mulhi(n, m) + n) >> s for
+// every 32-bit n (the sum is formed in 64 bits).
+struct Divisor {
+    uint32_t m;
+    int s;
+};
+
+inline Divisor make_divisor(uint32_t d) {
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    return {(uint32_t)((((1ull << 32) * ((1ull << s) - d)) / d) + 1), s};
+}
+
+__device__ inline uint32_t divide(uint32_t n, Divisor v) {
+    return (uint32_t)(((uint64_t)__umulhi(n, v.m) + n) >> v.s);
+}
+
 // Geometry of an implicit 3x3 (padding 1) convolution operand: the element for pixel p and
 // column j = tap * C + c is X[b, x + tap/3 - 1, y + tap%3 - 1, c] (zero off the image), with
 // p = (b*nx + x)*ny + y and X channels-last with channel stride ld (the operand's lda/ldb).
+// Pixel counts stay below 2^31.
 struct ConvGeom {
     int nx, ny, C;
+    Divisor dx, dy, dc;  // division by nx, ny and C
 };
+
+inline ConvGeom conv_geom(int nx, int ny, int C) {
+    return {nx, ny, C, make_divisor((uint32_t)nx), make_divisor((uint32_t)ny), make_divisor((uint32_t)C)};
+}
 
 struct GemmArgs {
     const u16* a;
@@ -56,15 +81,13 @@ struct GemmArgs {
 };
 
 // address of the 8 channels [c, c+8) of pixel p's neighbour through tap, or null off the image
-__device__ inline const u16* conv_src(const u16* X, int ld, const ConvGeom& g, long long p, int j) {
-    const int tap = j / g.C, c = j - tap * g.C;
-    const int y = (int)(p % g.ny);
-    const long long bx = p / g.ny;
-    const int x = (int)(bx % g.nx);
-    const long long b = bx / g.nx;
-    const int xx = x + tap / 3 - 1, yy = y + tap % 3 - 1;
+__device__ inline const u16* conv_src(const u16* X, int ld, const ConvGeom& g, uint32_t p, uint32_t j) {
+    const uint32_t tap = divide(j, g.dc), c = j - tap * (uint32_t)g.C;
+    const uint32_t bx = divide(p, g.dy), b = divide(bx, g.dx);
+    const int y = (int)(p - bx * (uint32_t)g.ny), x = (int)(bx - b * (uint32_t)g.nx);
+    const int xx = x + (int)(tap / 3) - 1, yy = y + (int)(tap % 3) - 1;
     if (xx < 0 || xx >= g.nx || yy < 0 || yy >= g.ny) return nullptr;
-    return X + ((size_t)(b * g.nx + xx) * g.ny + yy) * ld + c;
+    return X + (size_t)((b * (uint32_t)g.nx + (uint32_t)xx) * (uint32_t)g.ny + (uint32_t)yy) * ld + c;
 }
 
 constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8, kLdsRow = kTK + kPad;
