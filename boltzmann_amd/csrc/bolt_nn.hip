@@ -72,6 +72,7 @@ struct bolt_nn {
     // activations (bf16), per level: ea = first conv of the level, cat = [up | second conv]
     // (levels < depth), pin = pooled input (levels >= 1; pin[0] = x0), u = decoder output
     u16 *x0 = nullptr, *bb = nullptr;
+    float* xf = nullptr;   // float features of one batch as uploaded, before padding to bf16
     std::vector<u16*> ea, cat, pin, u;
     float* out;
     // gradients (float)
@@ -287,24 +288,24 @@ static int adam(bolt_nn* N, float lr) {
 }
 
 // host features [n][nx][ny][8] (float) -> device [n][nx][ny][16] bf16 (zero padded)
-static void pad_features(const float* x, size_t pixels, std::vector<uint16_t>& out) {
-    out.assign(pixels * kInC, 0);
-    for (size_t p = 0; p < pixels; ++p)
-        for (int c = 0; c < kFeat; ++c) {
-            float v = x[p * kFeat + c];
-            uint32_t u;
-            memcpy(&u, &v, 4);
-            u += 0x7fffu + ((u >> 16) & 1u);
-            out[p * kInC + c] = (uint16_t)(u >> 16);
-        }
+// Uploads the float features of `cnt` samples and pads them to bf16 [pixels][kInC] at dst
+// on the device; pixels of the samples after `cnt` (up to `slots` samples) are zeroed.
+static int upload_features(bolt_nn* N, const float* x, int cnt, int slots, u16* dst) {
+    const size_t pix = (size_t)N->nx * N->ny;
+    hipStream_t s = N->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(N->xf, x, (size_t)cnt * pix * kFeat * 4, hipMemcpyHostToDevice, s));
+    const long long n = (long long)slots * pix;
+    hipLaunchKernelGGL((pad_features<kFeat, kInC>), dim3(nblk(n * kInC)), dim3(256), 0, s, N->xf, n,
+                       (long long)cnt * pix, dst);
+    HIP_TRY(hipGetLastError());
+    return BOLT_OK;
 }
 
-static int upload_batch_inputs(bolt_nn* N, const float* x, const float* y) {
+static int upload_batch_inputs(bolt_nn* N, const float* x, int cnt, const float* y) {
     const size_t pix = (size_t)N->batch * N->nx * N->ny;
-    std::vector<uint16_t> xb;
-    pad_features(x, pix, xb);
+    int rc = upload_features(N, x, cnt, N->batch, N->x0);
+    if (rc) return rc;
     hipStream_t s = N->ctx->stream;
-    HIP_TRY(hipMemcpyAsync(N->x0, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, s));
     if (y) HIP_TRY(hipMemcpyAsync(N->target, y, pix * kOutC * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return BOLT_OK;
@@ -381,6 +382,7 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int dept
     n->dpin.assign(D + 1, nullptr);
     n->du.assign(D, nullptr);
     A((void**)&n->x0, P0 * kInC * 2);
+    A((void**)&n->xf, P0 * kFeat * 4);
     n->pin[0] = n->x0;
     for (int l = 0; l <= D; ++l) {
         const size_t P = P0 >> (2 * l);
@@ -461,7 +463,7 @@ int bolt_nn_loss_grad(bolt_nn* N, const float* x, const float* y, double* loss_o
     if (!N || !x || !y) return fail(BOLT_EINVAL, "null argument");
     if (grads && ng != N->nparam) return fail(BOLT_EINVAL, "expected %zu gradients", N->nparam);
     HIP_TRY(hipSetDevice(N->ctx->device));
-    int rc = upload_batch_inputs(N, x, y);
+    int rc = upload_batch_inputs(N, x, N->batch, y);
     if (!rc) rc = zero_grads(N);
     if (rc) return rc;
     forward(N);
@@ -489,12 +491,11 @@ int bolt_nn_set_data(bolt_nn* N, int n, const float* x, const float* y) {
     const size_t pix = (size_t)N->nx * N->ny;
     HIP_TRY(hipMalloc(&N->data_x, (size_t)n * pix * kInC * 2));
     HIP_TRY(hipMalloc(&N->data_y, (size_t)n * pix * kOutC * 4));
-    std::vector<uint16_t> xb;
-    for (int i = 0; i < n; ++i) {
-        pad_features(x + (size_t)i * pix * kFeat, pix, xb);
-        HIP_TRY(hipMemcpyAsync(N->data_x + (size_t)i * pix * kInC, xb.data(), xb.size() * 2,
-                               hipMemcpyHostToDevice, N->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(N->ctx->stream));  // xb is refilled next iteration
+    // a batch of samples at a time through the float staging buffer (stream-ordered reuse)
+    for (int i0 = 0; i0 < n; i0 += N->batch) {
+        const int cnt = std::min(N->batch, n - i0);
+        int rc = upload_features(N, x + (size_t)i0 * pix * kFeat, cnt, cnt, N->data_x + (size_t)i0 * pix * kInC);
+        if (rc) return rc;
     }
     HIP_TRY(hipMemcpyAsync(N->data_y, y, (size_t)n * pix * kOutC * 4, hipMemcpyHostToDevice, N->ctx->stream));
     HIP_TRY(hipStreamSynchronize(N->ctx->stream));
@@ -557,12 +558,9 @@ int bolt_nn_predict(bolt_nn* N, int n, const float* x, float* y) {
     if (!N || !x || !y || n < 1) return fail(BOLT_EINVAL, "bad arguments");
     HIP_TRY(hipSetDevice(N->ctx->device));
     const size_t pix = (size_t)N->nx * N->ny;
-    std::vector<float> xin((size_t)N->batch * pix * kFeat, 0.f);
     for (int i0 = 0; i0 < n; i0 += N->batch) {
         const int cnt = std::min(N->batch, n - i0);
-        std::fill(xin.begin(), xin.end(), 0.f);
-        memcpy(xin.data(), x + (size_t)i0 * pix * kFeat, (size_t)cnt * pix * kFeat * 4);
-        int rc = upload_batch_inputs(N, xin.data(), nullptr);
+        int rc = upload_batch_inputs(N, x + (size_t)i0 * pix * kFeat, cnt, nullptr);
         if (rc) return rc;
         forward(N);
         HIP_TRY(hipGetLastError());

@@ -393,6 +393,18 @@ __global__ void adam_step(float* __restrict__ w, u16* __restrict__ wb, const flo
     if (wb) wb[i] = bf16_bits(wi);
 }
 
+// Input features [pixels][FEAT] float -> [pixels][INC] bf16 with the channels from FEAT and
+// the pixels from `valid` on zero
+template <int FEAT, int INC>
+__global__ void pad_features(const float* __restrict__ x, long long pixels, long long valid,
+                             u16* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pixels * INC) return;
+    const long long p = t / INC;
+    const int c = (int)(t - p * INC);
+    out[t] = (c < FEAT && p < valid) ? bf16_bits(x[p * FEAT + c]) : (u16)0;
+}
+
 __global__ void to_bf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = bf16_bits(x[i]);

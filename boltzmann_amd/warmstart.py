@@ -13,7 +13,9 @@ max|u^n - u^(n-100)| / (100 max|u^n|) drops below the tolerance (1e-6).
 
 from __future__ import annotations
 
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -263,6 +265,16 @@ def from_targets(case: Case, t: np.ndarray):
     return rho, vel
 
 
+def _each(fn, items) -> list:
+    """fn over items on a thread pool (the numpy copies and scipy's distance transform release
+    the GIL); results in order."""
+    items = list(items)
+    if len(items) < 2:
+        return [fn(i) for i in items]
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1, len(items))) as ex:
+        return list(ex.map(fn, items))
+
+
 def features(cases) -> np.ndarray:
     """[n, nx, ny, 8] channels-last network inputs, per cell:
     0 wall mask, 1 u_in / 0.05, 2 (tau - 1/2) * 2, 3 Re / 100,
@@ -273,20 +285,30 @@ def features(cases) -> np.ndarray:
     from scipy.ndimage import distance_transform_cdt
     nx, ny = cases[0].nx, cases[0].ny
     out = np.empty((len(cases), nx, ny, 8), np.float32)
-    out[..., 4] = (np.arange(nx, dtype=np.float32) / nx)[None, :, None]
-    out[..., 5] = (np.arange(ny, dtype=np.float32) / ny)[None, None, :]
+    xs = (np.arange(nx, dtype=np.float32) / nx)[:, None]
+    ys = (np.arange(ny, dtype=np.float32) / ny)[None, :]
     sy = np.clip((np.arange(ny) - 0.5) / (ny - 2), 0.0, 1.0)
     prof = (6 * sy * (1 - sy)).astype(np.float32)
-    for k, c in enumerate(cases):
+
+    def fill(k):
+        # channel planes first, then one interleaving copy (3.5x faster than eight strided
+        # channel writes)
+        c = cases[k]
         assert (c.nx, c.ny) == (nx, ny), "every case of one call has the same size"
-        out[k, ..., 0] = c.mask
-        out[k, ..., 1] = c.u_in / 0.05
-        out[k, ..., 2] = (c.tau - 0.5) * 2.0
-        out[k, ..., 3] = c.re / 100.0
-        out[k, ..., 6] = distance_transform_cdt(~c.mask, metric="chessboard") / 16.0
+        planes = np.empty((8, nx, ny), np.float32)
+        planes[0] = c.mask
+        planes[1] = c.u_in / 0.05
+        planes[2] = (c.tau - 0.5) * 2.0
+        planes[3] = c.re / 100.0
+        planes[4] = xs
+        planes[5] = ys
+        planes[6] = distance_transform_cdt(~c.mask, metric="chessboard") / 16.0
         open_cells = np.maximum((~c.mask).sum(1), 1).astype(np.float32)
         lam = prof[None, :] * ((ny - 2) / open_cells)[:, None]
-        out[k, ..., 7] = np.where(c.mask, 0.0, lam)
+        planes[7] = np.where(c.mask, 0.0, lam)
+        np.copyto(out[k], planes.transpose(1, 2, 0))
+
+    _each(fill, range(len(cases)))
     return out
 
 
@@ -453,7 +475,7 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
     t0 = time.perf_counter()
     pred = net.predict_cases(cases, mirror) if hasattr(net, "predict_cases") else \
         net.predict(features(cases))
-    init = [from_targets(c, p) for c, p in zip(cases, pred)]
+    init = _each(lambda k: from_targets(cases[k], pred[k]), range(len(cases)))
     t_inf = time.perf_counter() - t0
     warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
                     dev=dev, keep_fields=False, neq=neq)
