@@ -120,7 +120,7 @@ static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, con
     g.ldc = ldc;
     g.c_off = c_off;
     g.relu = relu ? 1 : 0;
-    g.k_split = k_split > 0 ? k_split : ((K + kTK - 1) / kTK) * kTK;
+    g.k_split = k_split > 0 ? k_split : ((K + kStage - 1) / kStage) * kStage;
     const dim3 grid(nblk(M, kTM), nblk(Nn, kTN), (unsigned)((K + g.k_split - 1) / g.k_split));
     hipStream_t s = N->ctx->stream;
 #define BOLT_GEMM(A, B, O) hipLaunchKernelGGL((gemm_bf16<A, B, O>), grid, dim3(256), 0, s, g)
@@ -313,9 +313,10 @@ static int upload_batch_inputs(bolt_nn* N, const float* x, int cnt, const float*
 
 int bolt_nn_destroy(bolt_nn* N) {
     if (!N) return BOLT_OK;
-    hipSetDevice(N->ctx->device);
-    hipStreamSynchronize(N->ctx->stream);
-    for (auto& b : N->bufs) hipFree(b.p);
+    // teardown: errors here have no caller to act on them
+    (void)hipSetDevice(N->ctx->device);
+    (void)hipStreamSynchronize(N->ctx->stream);
+    for (auto& b : N->bufs) (void)hipFree(b.p);
     delete N;
     return BOLT_OK;
 }
@@ -483,8 +484,8 @@ int bolt_nn_set_data(bolt_nn* N, int n, const float* x, const float* y) {
     if (!N || !x || !y || n < 1) return fail(BOLT_EINVAL, "bad arguments");
     HIP_TRY(hipSetDevice(N->ctx->device));
     if (N->data_x) {
-        hipFree(N->data_x);
-        hipFree(N->data_y);
+        (void)hipFree(N->data_x);
+        (void)hipFree(N->data_y);
         N->data_x = nullptr;
         N->data_y = nullptr;
     }
@@ -597,20 +598,22 @@ int bolt_nn_gemm_check(bolt_ctx* ctx, int a_form, int b_form, int M, int Nn, int
         // stream-ordered before the GEMM (the context stream does not wait for the null
         // stream; a late zero-fill would wipe split-K partial sums)
         hipStream_t s = ctx->stream;
-        hipMemcpyAsync(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice, s);
-        hipMemsetAsync(dc, 0, (size_t)M * Nn * 4, s);
+        hipError_t e = hipMemcpyAsync(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemsetAsync(dc, 0, (size_t)M * Nn * 4, s);
         const int lda = a_form == kMK ? K : M, ldb = b_form == kNK ? K : Nn;
-        gemm(&tmp, a_form, b_form, k_split > 0 ? kAtomicF32 : kStoreF32, da, lda, db, ldb, dc, Nn, 0, M, Nn, K,
-             nullptr, false, k_split);
-        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) {
+            gemm(&tmp, a_form, b_form, k_split > 0 ? kAtomicF32 : kStoreF32, da, lda, db, ldb, dc, Nn, 0, M,
+                 Nn, K, nullptr, false, k_split);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c, dc, (size_t)M * Nn * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = fail(BOLT_EDEVICE, "gemm check: %s", hipGetErrorString(e));
     }
-    hipFree(da);
-    hipFree(db);
-    hipFree(dc);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
     return rc;
 }

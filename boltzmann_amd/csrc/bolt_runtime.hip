@@ -467,10 +467,10 @@ int bolt_ctx_create(int device_kind, int device_index, bolt_ctx** out) {
 
 int bolt_ctx_destroy(bolt_ctx* ctx) {
     if (!ctx) return BOLT_OK;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     if (ctx->stream) {
-        hipStreamSynchronize(ctx->stream);
-        hipStreamDestroy(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
     return BOLT_OK;
@@ -484,21 +484,21 @@ int bolt_ctx_device_info(const bolt_ctx* ctx, char* buf, size_t len) {
 
 int bolt_lattice_destroy(bolt_lattice* L) {
     if (!L) return BOLT_OK;
-    hipSetDevice(L->ctx->device);
-    hipStreamSynchronize(L->ctx->stream);
-    hipFree(L->f);
-    hipFree(L->rho);
-    hipFree(L->vel);
-    hipFree(L->flags);
+    (void)hipSetDevice(L->ctx->device);
+    (void)hipStreamSynchronize(L->ctx->stream);
+    (void)hipFree(L->f);
+    (void)hipFree(L->rho);
+    (void)hipFree(L->vel);
+    (void)hipFree(L->flags);
     for (auto& t : L->tracers) {
-        hipFree(t.g);
-        hipFree(t.C);
+        (void)hipFree(t.g);
+        (void)hipFree(t.C);
     }
-    hipFree(L->partials);
-    hipFree(L->result);
-    hipFree(L->vel_snap);
-    if (L->ev0) hipEventDestroy(L->ev0);
-    if (L->ev1) hipEventDestroy(L->ev1);
+    (void)hipFree(L->partials);
+    (void)hipFree(L->result);
+    (void)hipFree(L->vel_snap);
+    if (L->ev0) (void)hipEventDestroy(L->ev0);
+    if (L->ev1) (void)hipEventDestroy(L->ev1);
     delete L;
     return BOLT_OK;
 }
@@ -746,8 +746,8 @@ static int transfer(bolt_lattice* L, int field, int tracer_id, int host_dtype, v
         if (e != hipSuccess)
             status = fail(BOLT_EDEVICE, "field transfer failed: %s", hipGetErrorString(e));
     }
-    hipStreamSynchronize(s);
-    hipFree(stage);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(stage);
     return status;
 }
 
@@ -1169,7 +1169,7 @@ int bolt_lattice_postproc(bolt_lattice* L, int kind, const int* origin, float* h
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host, buf, want, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "postproc: %s", hipGetErrorString(e));
     return BOLT_OK;
 }
@@ -1266,7 +1266,7 @@ struct bolt_batch {
 };
 
 static void batch_drop_graph(bolt_batch* B) {
-    if (B->graph) hipGraphExecDestroy(B->graph);
+    if (B->graph) (void)hipGraphExecDestroy(B->graph);
     B->graph = nullptr;
     B->graph_block = B->graph_len = 0;
 }
@@ -1281,7 +1281,7 @@ static int batch_build_tiles(bolt_batch* B, int block) {
             for (int c = 0; c < cpr; ++c) tiles.push_back({(uint16_t)k, (uint16_t)r, (uint16_t)c, 0});
     }
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));  // no launch still reads the old table
-    hipFree(B->d_tiles);
+    (void)hipFree(B->d_tiles);
     B->d_tiles = nullptr;
     if (!tiles.empty()) {
         HIP_TRY(hipMalloc(&B->d_tiles, tiles.size() * sizeof(Tile)));
@@ -1318,7 +1318,7 @@ static int batch_capture(bolt_batch* B, int len) {
     for (int k = 0; k < len; ++k) batch_launch(B, (k % 2) == 0, false, s);
     HIP_TRY(hipStreamEndCapture(s, &g));
     hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     B->graph_block = B->block;
     B->graph_len = len;
@@ -1377,7 +1377,7 @@ static int batch_upload_desc(bolt_batch* B) {
             d.gy = L->gravity[1];
         }
     }
-    hipFree(B->d_inst);
+    (void)hipFree(B->d_inst);
     B->d_inst = nullptr;
     HIP_TRY(hipMalloc(&B->d_inst, h.size() * sizeof(InstDesc)));
     HIP_TRY(hipMemcpyAsync(B->d_inst, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice,
@@ -1430,14 +1430,14 @@ int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision
 
 int bolt_batch_destroy(bolt_batch* B) {
     if (!B) return BOLT_OK;
-    hipSetDevice(B->ctx->device);
-    hipStreamSynchronize(B->ctx->stream);
+    (void)hipSetDevice(B->ctx->device);
+    (void)hipStreamSynchronize(B->ctx->stream);
     batch_drop_graph(B);
     for (auto* L : B->lat) bolt_lattice_destroy(L);
-    hipFree(B->d_inst);
-    hipFree(B->d_tiles);
-    if (B->ev0) hipEventDestroy(B->ev0);
-    if (B->ev1) hipEventDestroy(B->ev1);
+    (void)hipFree(B->d_inst);
+    (void)hipFree(B->d_tiles);
+    if (B->ev0) (void)hipEventDestroy(B->ev0);
+    if (B->ev1) (void)hipEventDestroy(B->ev1);
     delete B;
     return BOLT_OK;
 }
@@ -1573,12 +1573,15 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
             }
         }
     }
-    for (auto& sv : saved) {
-        hipMemcpyAsync(sv.dst, sv.src, sv.bytes, hipMemcpyDeviceToDevice, s);
-    }
-    hipStreamSynchronize(s);
-    for (auto& sv : saved) hipFree(sv.src);
+    // restore the state the candidates stepped; a failed restore is reported, since the
+    // batch would otherwise continue from a tuning run's state
+    hipError_t e = hipSuccess;
+    for (auto& sv : saved)
+        if (e == hipSuccess) e = hipMemcpyAsync(sv.dst, sv.src, sv.bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (auto& sv : saved) (void)hipFree(sv.src);
     if (rc) return rc;
+    if (e != hipSuccess) return fail(BOLT_EDEVICE, "tune: restoring the batch state failed: %s", hipGetErrorString(e));
     rc = bolt_batch_set_shape(B, best_block, best_len);
     char line[96];
     snprintf(line, sizeof line, "chosen: block %d graph %d\n", best_block, best_len);

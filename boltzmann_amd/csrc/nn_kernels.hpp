@@ -10,7 +10,7 @@
 //     (n = tap*C + c, k = pixel),
 // which covers the forward convolution (MKconv(X) x W^T), the input gradient (MKconv(dZ) x
 // Wt^T with the reflected, transposed kernel Wt) and the weight gradient (dZ^T x KNconv(X)),
-// with no im2col buffer.  Work-group tile 64 x 64, K step 32, four waves each owning a 32 x 32
+// with no im2col buffer.  Work-group tile 64 x 64, K step 64, four waves each owning a 32 x 32
 // block computed with v_mfma_f32_32x32x16_bf16, float accumulation.
 #pragma once
 
@@ -90,7 +90,15 @@ __device__ inline const u16* conv_src(const u16* X, int ld, const ConvGeom& g, u
     return X + (size_t)((b * (uint32_t)g.nx + (uint32_t)xx) * (uint32_t)g.ny + (uint32_t)yy) * ld + c;
 }
 
-constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8, kLdsRow = kTK + kPad;
+// element e (0..7) of 8 packed bf16 values, by shifts: a pointer cast of the vector would
+// place it in scratch memory
+__device__ inline uint32_t half16(const uint4& q, int e) {
+    const uint32_t w = e < 2 ? q.x : e < 4 ? q.y : e < 6 ? q.z : q.w;
+    return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// kTK: the K width one tile load fills; kStage: the K width staged in LDS per barrier pair
+constexpr int kTM = 64, kTN = 64, kTK = 32, kStage = 64, kPad = 8, kLdsRow = kStage + kPad;
 
 // Loads a 64 x 32 tile of an operand into LDS as [row][k] (k contiguous), zero-filled
 // outside the matrix.  kform: 0 -> element (row, k) at p[row*ld + k]; 1 -> at p[k*ld + row].
@@ -120,9 +128,8 @@ __device__ inline void load_tile(u16* lds, const u16* __restrict__ p, int ld, in
             const u16* src = conv_src(p, ld, cg, gk, gr);
             if (src) q = *(const uint4*)src;
         }
-        const u16* v = (const u16*)&q;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) lds[(rc + e) * kLdsRow + k] = v[e];
+        for (int e = 0; e < 8; ++e) lds[(rc + e) * kLdsRow + k] = (u16)half16(q, e);
         return;
     }
     if (FORM == 0) {
@@ -168,10 +175,14 @@ __device__ inline uint4 load8_kmajor(const u16* __restrict__ p, int ld, int r, i
         return q;
     }
     if (r + 8 <= rows && (ld & 7) == 0) return *(const uint4*)(p + (size_t)k * ld + r);
-    u16* v = (u16*)&q;
+    uint32_t w[4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (r + e < rows) ? p[(size_t)k * ld + r + e] : (u16)0;
-    return q;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t lo = (r + 2 * i < rows) ? p[(size_t)k * ld + r + 2 * i] : 0u;
+        const uint32_t hi = (r + 2 * i + 1 < rows) ? p[(size_t)k * ld + r + 2 * i + 1] : 0u;
+        w[i] = lo | (hi << 16);
+    }
+    return {w[0], w[1], w[2], w[3]};
 }
 
 // Fills one 64 x 32 LDS tile ([row][k]) of a k-major operand with 128 threads: thread t
@@ -183,11 +194,9 @@ __device__ inline void load_tile_kpairs(u16* lds, const u16* __restrict__ p, int
     const int kp = 2 * (t >> 3), rc = (t & 7) * 8;
     const uint4 q0 = load8_kmajor<FORM>(p, ld, row0 + rc, rows, k0 + kp, kend, cg);
     const uint4 q1 = load8_kmajor<FORM>(p, ld, row0 + rc, rows, k0 + kp + 1, kend, cg);
-    const u16* v0 = (const u16*)&q0;
-    const u16* v1 = (const u16*)&q1;
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-        *(uint32_t*)(lds + (rc + e) * kLdsRow + kp) = (uint32_t)v0[e] | ((uint32_t)v1[e] << 16);
+        *(uint32_t*)(lds + (rc + e) * kLdsRow + kp) = half16(q0, e) | (half16(q1, e) << 16);
 }
 
 template <int AF, int BF, int OUT>
@@ -200,24 +209,29 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc = {0};
-    for (int k0 = kb; k0 < ke; k0 += kTK) {
+    for (int k0 = kb; k0 < ke; k0 += kStage) {
         bool pairs = false;
         if constexpr (AF == kKM && (BF == kKN || BF == kKNconv)) pairs = g.M >= 16;
-        if (pairs) {
-            // both operands k-major (the weight gradients of the 3x3 layers): half the threads
-            // fill each tile; a skinny A (the 1x1 output layer, M = 3) keeps one k per thread
-            if (threadIdx.x < 128)
-                load_tile_kpairs<1>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv, threadIdx.x);
-            else
-                load_tile_kpairs<BF == kKNconv ? 3 : 1>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv,
-                                                        threadIdx.x - 128);
-        } else {
-            load_tile<AF>(sa, g.a, g.lda, m0, g.M, k0, ke, g.conv);
-            load_tile<BF == kKNconv ? 3 : BF>(sb, g.b, g.ldb, n0, g.N, k0, ke, g.conv);
+#pragma unroll
+        for (int h = 0; h < kStage; h += kTK) {
+            // columns past ke (the slice end) load as zeros
+            if (pairs) {
+                // both operands k-major (the weight gradients of the 3x3 layers): half the
+                // threads fill each tile; a skinny A (the 1x1 output layer, M = 3) keeps one k
+                // per thread
+                if (threadIdx.x < 128)
+                    load_tile_kpairs<1>(sa + h, g.a, g.lda, m0, g.M, k0 + h, ke, g.conv, threadIdx.x);
+                else
+                    load_tile_kpairs<BF == kKNconv ? 3 : 1>(sb + h, g.b, g.ldb, n0, g.N, k0 + h, ke,
+                                                            g.conv, threadIdx.x - 128);
+            } else {
+                load_tile<AF>(sa + h, g.a, g.lda, m0, g.M, k0 + h, ke, g.conv);
+                load_tile<BF == kKNconv ? 3 : BF>(sb + h, g.b, g.ldb, n0, g.N, k0 + h, ke, g.conv);
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < kTK; kk += 16) {
+        for (int kk = 0; kk < kStage; kk += 16) {
             // A(i, k): lane holds row i = lane % 32, k = 8 * (lane / 32) .. + 7
             const bf16x8 a = *(const bf16x8*)(sa + (wm + (lane & 31)) * kLdsRow + kk + 8 * (lane >> 5));
             const bf16x8 b = *(const bf16x8*)(sb + (wn + (lane & 31)) * kLdsRow + kk + 8 * (lane >> 5));

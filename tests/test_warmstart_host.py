@@ -149,3 +149,31 @@ def test_mirrored_features_equal_the_reflected_cases_features():
     x = ws.features(cases)
     reflected = [ws.mirrored(c, *c.cold_state())[0] for c in cases]
     np.testing.assert_array_equal(ws.mirrored_features(x), ws.features(reflected))
+
+
+def test_features_of_a_batch_are_each_cases_own():
+    """features() fills the cases on a thread pool: every case's block must be what the case
+    gets alone, and the channels their definitions."""
+    from scipy.ndimage import distance_transform_cdt
+    cases = [ws.make_case(40 + s, nx=128, ny=64, size_range=(4, 10)) for s in range(6)]
+    x = ws.features(cases)
+    for k, c in enumerate(cases):
+        np.testing.assert_array_equal(x[k], ws.features([c])[0])
+        np.testing.assert_array_equal(x[k, ..., 0], c.mask.astype(np.float32))
+        np.testing.assert_array_equal(x[k, ..., 3], np.float32(c.re / 100.0))
+        np.testing.assert_array_equal(
+            x[k, ..., 6], (distance_transform_cdt(~c.mask, metric="chessboard") / 16.0).astype(np.float32))
+        lam = x[k, ..., 7]
+        assert (lam[c.mask] == 0).all()
+        # a column with no obstacle (open except its two wall cells) holds the plain laminar
+        # profile, whose mean over the open cells is 1 up to sampling the parabola at cell
+        # centres; blocked columns are scaled up by (ny - 2) / open cells
+        free = (~c.mask).sum(1) == c.ny - 2
+        assert free.any() and (~free).any()
+        prof = lam[free][0]
+        np.testing.assert_array_equal(lam[free], np.broadcast_to(prof, lam[free].shape))
+        assert abs(prof.sum() / (c.ny - 2) - 1) < 3e-3
+        blocked = np.flatnonzero(~free)[0]
+        open_cells = (~c.mask[blocked]).sum()
+        np.testing.assert_allclose(lam[blocked][~c.mask[blocked]],
+                                   prof[~c.mask[blocked]] * (c.ny - 2) / open_cells, rtol=1e-6)
