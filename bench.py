@@ -368,6 +368,10 @@ def main() -> int:
             r = run_gpu(n, coll, st, steps, args.warmup, rho, vel)
             v = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                  "hbm_frac": round(r["mlups"] * 1e6 * B_ALG[st] / (PEAK_HBM_GBS * 1e9), 4)}
+            if st in ("f16s", "bf16s") and ceilings and "shift27_h" in ceilings:
+                gbs = r["mlups"] * 1e6 * B_ALG[st] / 1e9
+                v["frac_of_aa_pair_ceiling"] = round(
+                    gbs * 0.5 * (1 / ceilings["inplace27_h"] + 1 / ceilings["shift27_h"]), 4)
             if coll == "trt":
                 e = run_gpu(n, coll, st, max(2, min(steps, 200)), 0, rho, vel, keep_vel=True)
                 v["u_err_vs_f64_200steps"] = float(np.abs(e["vel"] - ref64["vel"]).max() / umax)

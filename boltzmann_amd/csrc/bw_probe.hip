@@ -10,6 +10,7 @@
 //                 access mix of the step that streams, where 18 of the 27 accesses of a
 //                 wavefront start one element off a 256-byte boundary
 //   inplace27_h   the same as inplace27 with 2-byte elements (the f16s/bf16s storage)
+//   shift27_h     the same as shift27 with 2-byte elements
 //   inplace9      9 float planes in place over 3x the cells (the D2Q9 mix)
 // (each in-place figure the better of plain and streaming loads) and prints one JSON
 // object of GB/s (bytes read + written per second).  Every index is
@@ -69,11 +70,11 @@ __global__ void inplace_planes(T* __restrict__ f, long long n) {
 
 // Each element (q, j) is read and written by exactly one thread: for a fixed q the map
 // i -> (i + shift_q) mod n is a bijection.  n is a power of two, so the wrap is a mask.
-template <bool NT>
-__global__ void shifted_planes(float* __restrict__ f, long long n) {
+template <typename T, bool NT>
+__global__ void shifted_planes(T* __restrict__ f, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float v[27];
+    T v[27];
     long long a[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
@@ -128,13 +129,16 @@ int main(int argc, char** argv) {
     auto best = [](double a, double c) { return a > c ? a : c; };
     double r_in = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n); }, lat, reps),
                        gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n); }, lat, reps));
-    double r_sh = best(gbs([&] { shifted_planes<false><<<grid(n), b>>>(f, n); }, lat, reps),
-                       gbs([&] { shifted_planes<true><<<grid(n), b>>>(f, n); }, lat, reps));
+    double r_sh = best(gbs([&] { shifted_planes<float, false><<<grid(n), b>>>(f, n); }, lat, reps),
+                       gbs([&] { shifted_planes<float, true><<<grid(n), b>>>(f, n); }, lat, reps));
     // 2-byte elements: 27 planes of n halves occupy half of the first lattice
     const double half = 2.0 * 2.0 * (double)total;
     double r_inh = best(
         gbs([&] { inplace_planes<27, uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps),
         gbs([&] { inplace_planes<27, uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps));
+    double r_shh = best(
+        gbs([&] { shifted_planes<uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps),
+        gbs([&] { shifted_planes<uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps));
     // 9 planes over 3n cells: the same 27n floats
     double r_in9 = best(
         gbs([&] { inplace_planes<9, float, false><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps),
@@ -144,8 +148,8 @@ int main(int argc, char** argv) {
     CK(hipGetDeviceProperties(&p, 0));
     printf("{\"device\": \"%s\", \"bytes_per_pass\": %.0f, \"copy4\": %.1f, \"read27\": %.1f, "
            "\"write27\": %.1f, \"inplace27\": %.1f, \"shift27\": %.1f, \"inplace27_h\": %.1f, "
-           "\"inplace9\": %.1f}\n",
-           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_in9);
+           "\"shift27_h\": %.1f, \"inplace9\": %.1f}\n",
+           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_shh, r_in9);
     CK(hipFree(f));
     CK(hipFree(sink));
     return 0;
