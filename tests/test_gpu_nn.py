@@ -22,8 +22,10 @@ def bf16_round(a):
 
 
 @pytest.mark.parametrize("a_form,b_form", [(0, 0), (0, 1), (1, 0), (1, 1)])
+# (24, 200, 500): a 64-row tile with 40 empty rows, and K slices of two stages each
 @pytest.mark.parametrize("shape,k_split", [((200, 70, 100), 0), ((64, 64, 64), 0),
-                                           ((37, 129, 300), 64), ((3, 32, 1000), 96)])
+                                           ((37, 129, 300), 64), ((3, 32, 1000), 96),
+                                           ((24, 200, 500), 128)])
 def test_gemm_matches_numpy(a_form, b_form, shape, k_split):
     import boltzmann_amd
     M, N, K = shape
@@ -166,3 +168,27 @@ def test_training_reduces_loss():
     assert losses[-1] < 0.5 * losses[0], losses
     pred = net.predict(x[:5])
     assert pred.shape == (5, nx, ny, 3) and np.isfinite(pred).all()
+
+
+def test_prediction_does_not_depend_on_batching():
+    """Samples never mix (the 3x3 gathers stay inside each image), so predicting n samples in
+    batches, the last one partial, must give each sample what predicting it alone gives: this
+    checks the feature upload offsets and the zero padding of the unused batch slots.  A
+    sample's pixels can sit at other rows of a GEMM tile in the two runs, so the bound allows
+    a bf16 rounding flip in the activations; a wrong offset changes the outputs by O(1)."""
+    import boltzmann_amd
+    nx, ny, B = 32, 16, 4
+    net = boltzmann_amd.Network("gpu", nx, ny, B, width=16, seed=3)
+    x = np.random.default_rng(3).standard_normal((7, nx, ny, 8)).astype(np.float32)
+    together = net.predict(x)
+    assert together.shape == (7, nx, ny, 3) and np.isfinite(together).all()
+    scale = np.abs(together).max()
+    worst = 0.0
+    for i in range(7):
+        alone = net.predict(x[i:i + 1])[0]
+        worst = max(worst, float(np.abs(alone - together[i]).max()))
+    print("largest difference alone vs batched:", worst, "of", scale, flush=True)
+    # the bound must be tight enough to tell samples apart
+    apart = min(float(np.abs(together[i] - together[0]).max()) for i in range(1, 7))
+    print("smallest difference between samples:", apart, flush=True)
+    assert worst <= 2e-2 * scale < apart
