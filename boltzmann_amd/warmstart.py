@@ -7,7 +7,7 @@ rho = 1; like every fixed cell of the reference's kernels it also keeps its init
 velocity, lib.cl:257-262); 1 to 8 circles or rectangles of radius / half-size 8..32 as
 walls.  Re = u_in D / nu on D = twice the largest obstacle size sets tau = 3 nu + 1/2.
 
-The plain solver is the judge: a run has converged when the FP64 residual
+The plain solver decides convergence: a run has converged when the FP64 residual
 max|u^n - u^(n-100)| / (100 max|u^n|) drops below the tolerance (1e-6).
 """
 

@@ -3,7 +3,7 @@
 A reduced version of the steady-channel workload (128 x 64 channels, small obstacles, low
 Reynolds numbers so every case is steady): the solver generates training states, the
 network (depth-5 U-Net) is trained on the GPU with this library's kernels, and on held-out
-geometries the plain LBM -- the judge -- is run to the same FP64 residual from the cold
+geometries the plain LBM, which decides convergence, is run to the same FP64 residual from the cold
 start and from the network's prediction (averaged with its mirror image).
 The warm start must need fewer steps; a rejected prediction falls back to the cold start.
 """
