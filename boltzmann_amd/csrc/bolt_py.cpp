@@ -563,6 +563,20 @@ PYBIND11_MODULE(_boltzmann, m) {
             check(bolt_lattice_residual(s.core->lat, steps_between, &r), "residual");
             return r;
         }, py::arg("steps_between"))
+        .def("count_nonfinite", [](PySimulation& s, const std::string& field) {
+            // non-finite values of the device rho or vel after the last iterate, counted on
+            // the device (the host check of simulation.py:241-242 without its pass over the
+            // host array)
+            std::lock_guard<std::mutex> lk(s.core->mu);
+            if (s.core->iteration == 0) throw BoltError("count_nonfinite reads the device fields: call it after iterate");
+            int fid;
+            if (field == "vel") fid = BOLT_FIELD_VEL;
+            else if (field == "rho") fid = BOLT_FIELD_RHO;
+            else throw BoltError("count_nonfinite: field must be 'vel' or 'rho'");
+            uint64_t n = 0;
+            check(bolt_lattice_count_nonfinite(s.core->lat, fid, &n), "count_nonfinite");
+            return n;
+        }, py::arg("field") = "vel")
         .def("postproc", [](PySimulation& s, const std::string& what, py::object origin) {
             // utils/postproc.py:12-127 on the device velocity (lattice units)
             std::lock_guard<std::mutex> lk(s.core->mu);

@@ -371,6 +371,17 @@ __global__ void k_sum_momentum(const R* rho, const R* vel, const uint8_t* flags,
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// count of the non-finite values of x[0, n) per block, as a double (exact below 2^53)
+template <typename R>
+__global__ void k_count_nonfinite(const R* x, long long n, double* partials) {
+    double acc = 0.0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        acc += __builtin_isfinite((double)x[k]) ? 0.0 : 1.0;
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 __global__ void k_sum_partials(const double* partials, int n, double* out) {
     double acc = 0.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partials[k];
@@ -882,6 +893,25 @@ static int finish_reduce(bolt_lattice* L, double* out) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return BOLT_OK;
+}
+
+int bolt_lattice_count_nonfinite(bolt_lattice* L, int field, uint64_t* count) {
+    if (!L || !count) return fail(BOLT_EINVAL, "null argument");
+    if (field != BOLT_FIELD_RHO && field != BOLT_FIELD_VEL)
+        return fail(BOLT_EINVAL, "count_nonfinite reads BOLT_FIELD_RHO or BOLT_FIELD_VEL");
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    const long long n = (long long)L->ncell * (field == BOLT_FIELD_VEL ? L->ndim : 1);
+    const void* x = field == BOLT_FIELD_VEL ? L->vel : L->rho;
+    hipStream_t s = L->ctx->stream;
+    if (is_f64(L))
+        hipLaunchKernelGGL(k_count_nonfinite<double>, dim3(kRedBlocks), dim3(256), 0, s, (const double*)x, n, L->partials);
+    else
+        hipLaunchKernelGGL(k_count_nonfinite<float>, dim3(kRedBlocks), dim3(256), 0, s, (const float*)x, n, L->partials);
+    double r = 0.0;
+    const int rc = finish_reduce(L, &r);
+    if (rc) return rc;
+    *count = (uint64_t)r;
     return BOLT_OK;
 }
 

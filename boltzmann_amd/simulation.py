@@ -22,7 +22,6 @@ from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Generator
 
-import numpy as np
 
 logger = logging.getLogger(__name__)
 
@@ -110,7 +109,9 @@ def parse_checkpoints(s: str) -> CheckpointGater:
 def run_sim(sim, meta: IterInfo, output_dir: Path | str,
             checkpoints: CheckpointGater | None = None) -> Generator[int, None, None]:
     """Steps ``sim`` in batches of ``meta.interval`` (made even) up to ``meta.count`` outputs,
-    yielding the output index before the first batch and after each one."""
+    yielding the output index before the first batch and after each one.  After each batch
+    a non-finite velocity raises ValueError, as in simulation.py:241-242; the values are
+    counted on the device rather than by a pass over the host array."""
     output_dir = Path(output_dir)
     if output_dir.exists():
         if not output_dir.is_dir():
@@ -128,7 +129,7 @@ def run_sim(sim, meta: IterInfo, output_dir: Path | str,
     for i in range(start + 1, meta.count):
         t0 = time.perf_counter()
         sim.iterate(batch_iters)
-        if np.any(~np.isfinite(sim.fluid.vel)):
+        if sim.count_nonfinite("vel"):
             raise ValueError("Non-finite value detected!")
         secs = time.perf_counter() - t0
         mlups = sim.cells.count * batch_iters / (1e6 * secs)

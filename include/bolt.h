@@ -150,6 +150,12 @@ int bolt_lattice_reduce(bolt_lattice* lat, int what, int tracer_id, double* out)
  * completed bolt_lattice_step. */
 int bolt_lattice_residual(bolt_lattice* lat, uint64_t steps_between, double* out);
 
+/* Number of non-finite values of the device rho (BOLT_FIELD_RHO) or velocity (BOLT_FIELD_VEL)
+ * of the last completed bolt_lattice_step.  Replaces, on the device, the host check of
+ * python/boltzmann/simulation.py:241-242 (np.any(~np.isfinite(sim.fluid.vel)) after each
+ * iterate of run_sim); it has no counterpart in the reference's native code. */
+int bolt_lattice_count_nonfinite(bolt_lattice* lat, int field, uint64_t* count);
+
 /* Post-processing of the device velocity into a host float array, with the semantics of
  * python/boltzmann/utils/postproc.py:12-127 on the float32 velocity (np.gradient: central
  * differences inside, one-sided at the ends of each axis; sequential float32 cumsum), in

@@ -76,3 +76,25 @@ def test_postproc_edges_and_errors():
     with pytest.raises(RuntimeError, match="at least 2"):
         postproc.calc_curl(thin)
     np.testing.assert_array_equal(postproc.calc_vmag(thin), ref_vmag(thin.fluid.vel))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_count_nonfinite_matches_the_host_check(storage):
+    """count_nonfinite counts on the device what np.isfinite finds in the read-back arrays
+    (the check of the reference's run_sim, simulation.py:241-242)."""
+    import boltzmann_amd
+    from boltzmann_amd.core import bgk
+    sim = boltzmann_amd.Simulation("gpu", [24, 16], bgk(0.6), storage=storage)
+    sim.fluid.vel[..., 0] = 0.01
+    sim.iterate(2)
+    assert sim.count_nonfinite("vel") == 0 and sim.count_nonfinite("rho") == 0
+    bad = boltzmann_amd.Simulation("gpu", [24, 16], bgk(0.6), storage=storage)
+    bad.fluid.vel[5, 7, 0] = np.nan
+    bad.fluid.rho[11, 3] = np.inf
+    bad.iterate(4)                      # the non-finite values spread to the neighbours
+    for field, host in (("vel", bad.fluid.vel), ("rho", bad.fluid.rho)):
+        n = bad.count_nonfinite(field)
+        assert n > 2, (field, n)
+        assert n == np.count_nonzero(~np.isfinite(host)), field
+    with pytest.raises(boltzmann_amd.BoltError):
+        bad.count_nonfinite("C")
