@@ -78,6 +78,27 @@ struct CouplingH {
     std::vector<float> gravity;
 };
 
+// Page-locked host storage (bolt_host_alloc) for the arrays read back after every iterate,
+// so the copies run by DMA at the link's rate rather than through staged pageable copies
+template <class T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U>
+    PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (bolt_host_alloc(n * sizeof(T), &p) != BOLT_OK || (!p && n)) throw std::bad_alloc();
+        return (T*)p;
+    }
+    void deallocate(T* p, size_t) { (void)bolt_host_free(p); }
+    template <class U>
+    bool operator==(const PinnedAlloc<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+using PinnedVec = std::vector<float, PinnedAlloc<float>>;
+
 // Host state of one simulation.  The host arrays never change size, so numpy views of
 // them stay valid for the lifetime of the object.
 struct Core {
@@ -89,7 +110,8 @@ struct Core {
     OmegaV omega;
     int storage = BOLT_STORE_F32;
     std::vector<int32_t> flags;
-    std::vector<float> f, rho, vel;
+    std::vector<float> f;
+    PinnedVec rho, vel;
     std::vector<std::unique_ptr<TracerH>> tracers;
     bool has_gravity = false;
     std::vector<float> gravity;
@@ -139,9 +161,9 @@ std::shared_ptr<Core> make_core(const std::string& dev, std::vector<int64_t> cou
     return c;
 }
 
-template <typename T>
-py::array view(const std::shared_ptr<Core>& core, std::vector<T>& v,
-               std::vector<ssize_t> shape) {
+template <typename V>
+py::array view(const std::shared_ptr<Core>& core, V& v, std::vector<ssize_t> shape) {
+    using T = typename V::value_type;
     std::vector<ssize_t> strides(shape.size());
     ssize_t s = sizeof(T);
     for (int i = (int)shape.size() - 1; i >= 0; --i) {
@@ -425,8 +447,9 @@ PYBIND11_MODULE(_boltzmann, m) {
             st.counts = c.counts;
             st.flags = &c.flags;
             st.f = &c.f;
-            st.rho = &c.rho;
-            st.vel = &c.vel;
+            const std::vector<float> rho_h(c.rho.begin(), c.rho.end()), vel_h(c.vel.begin(), c.vel.end());
+            st.rho = &rho_h;
+            st.vel = &vel_h;
             st.q = c.q;
             st.omega_kind = c.omega.kind;
             st.omega_a = c.omega.a;
@@ -454,8 +477,8 @@ PYBIND11_MODULE(_boltzmann, m) {
             Core& c = *core;
             c.flags = std::move(ld.flags);
             c.f = std::move(ld.f);
-            c.rho = std::move(ld.rho);
-            c.vel = std::move(ld.vel);
+            c.rho.assign(ld.rho.begin(), ld.rho.end());
+            c.vel.assign(ld.vel.begin(), ld.vel.end());
             for (auto& t : ld.tracers) {
                 OmegaV to;
                 to.kind = t.omega_kind;

@@ -896,6 +896,19 @@ static int finish_reduce(bolt_lattice* L, double* out) {
     return BOLT_OK;
 }
 
+int bolt_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(BOLT_EINVAL, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return BOLT_OK;
+    HIP_TRY(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    return BOLT_OK;
+}
+
+int bolt_host_free(void* p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return BOLT_OK;
+}
+
 int bolt_lattice_count_nonfinite(bolt_lattice* L, int field, uint64_t* count) {
     if (!L || !count) return fail(BOLT_EINVAL, "null argument");
     if (field != BOLT_FIELD_RHO && field != BOLT_FIELD_VEL)

@@ -150,6 +150,13 @@ int bolt_lattice_reduce(bolt_lattice* lat, int what, int tracer_id, double* out)
  * completed bolt_lattice_step. */
 int bolt_lattice_residual(bolt_lattice* lat, uint64_t steps_between, double* out);
 
+/* Page-locked host memory for a caller's field arrays, so that bolt_lattice_read and
+ * bolt_lattice_write move them by DMA at the link's full rate (the reference's host arrays,
+ * lib.rs:57-67, are ordinary memory; this has no counterpart there).  bolt_host_free(NULL)
+ * does nothing. */
+int bolt_host_alloc(size_t bytes, void** out);
+int bolt_host_free(void* p);
+
 /* Number of non-finite values of the device rho (BOLT_FIELD_RHO) or velocity (BOLT_FIELD_VEL)
  * of the last completed bolt_lattice_step.  Replaces, on the device, the host check of
  * python/boltzmann/simulation.py:241-242 (np.any(~np.isfinite(sim.fluid.vel)) after each
