@@ -141,7 +141,20 @@ def cpu_baseline(budget_s: float = 12.0):
     return {"value": n ** 3 * steps / (1e6 * dt), "unit": "MLUPS", "cores": oracle.num_threads(),
             "kind": "port",
             "sample": f"oracle (OpenMP C++ restatement of lib.cl) D3Q27 TRT TGV {n}^3, "
-                      f"{steps} steps, {dt:.1f} s"}
+                      f"{steps} steps, {dt:.1f} s",
+            "cpu": _cpu_model(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def _cpu_model() -> str:
+    """The host CPU's model name (first entry of /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def rb_batch(steps: int = 50000):
