@@ -64,20 +64,6 @@ int storage_from(const std::string& s) {
     throw BoltError("unknown storage '" + s + "' (f32, f16s, bf16s, f64, f32f)");
 }
 
-struct TracerH {
-    std::string name;
-    int q = 0;
-    OmegaV omega;
-    int dev_id = -1;
-    std::vector<float> g, C;
-};
-
-struct CouplingH {
-    std::string tracer;
-    float alpha = 0.f, c0 = 0.f;
-    std::vector<float> gravity;
-};
-
 // Page-locked host storage (bolt_host_alloc) for the arrays read back after every iterate,
 // so the copies run by DMA at the link's rate rather than through staged pageable copies
 template <class T>
@@ -98,6 +84,21 @@ struct PinnedAlloc {
     bool operator!=(const PinnedAlloc<U>&) const { return false; }
 };
 using PinnedVec = std::vector<float, PinnedAlloc<float>>;
+
+struct TracerH {
+    std::string name;
+    int q = 0;
+    OmegaV omega;
+    int dev_id = -1;
+    std::vector<float> g;
+    PinnedVec C;
+};
+
+struct CouplingH {
+    std::string tracer;
+    float alpha = 0.f, c0 = 0.f;
+    std::vector<float> gravity;
+};
 
 // Host state of one simulation.  The host arrays never change size, so numpy views of
 // them stay valid for the lifetime of the object.
@@ -455,8 +456,12 @@ PYBIND11_MODULE(_boltzmann, m) {
             st.omega_a = c.omega.a;
             st.omega_b = c.omega.b;
             st.has_acc = c.has_gravity || !c.couplings.empty();
-            for (auto& t : c.tracers)
-                st.tracers.push_back({t->name, t->q, t->omega.kind, t->omega.a, t->omega.b, &t->g, &t->C});
+            std::vector<std::vector<float>> conc;  // the tracers' C as the serialiser takes them
+            conc.reserve(c.tracers.size());
+            for (auto& t : c.tracers) {
+                conc.emplace_back(t->C.begin(), t->C.end());
+                st.tracers.push_back({t->name, t->q, t->omega.kind, t->omega.a, t->omega.b, &t->g, &conc.back()});
+            }
             st.has_gravity = c.has_gravity;
             st.gravity = c.gravity;
             for (auto& cp : c.couplings) st.couplings.push_back({cp.tracer, cp.alpha, cp.c0, cp.gravity});
@@ -486,7 +491,7 @@ PYBIND11_MODULE(_boltzmann, m) {
                 to.b = t.omega_b;
                 add_tracer_core(c, t.name, to, t.q);
                 c.tracers.back()->g = std::move(t.g);
-                c.tracers.back()->C = std::move(t.C);
+                c.tracers.back()->C.assign(t.C.begin(), t.C.end());
             }
             if (ld.has_gravity) {
                 c.has_gravity = true;
