@@ -146,12 +146,14 @@ static void gemm(bolt_nn* N, int af, int bf, int out, const u16* a, int lda, con
 #undef BOLT_GEMM
 }
 
-// dbias += column sums of dz [P][C]: up to 1024 work-groups of contiguous row chunks
-static void bias_grad_launch(bolt_nn* N, long long P, int C, float* db) {
+// dz = the layer's pre-activation gradient (bf16) and dbias += its column sums: up to 1024
+// work-groups of contiguous row chunks
+static void relu_bias_grad_launch(bolt_nn* N, const float* dY, int lddy, int dyoff, const u16* Y, int ldy,
+                                  int yoff, long long P, int C, bool relu, float* db) {
     const long long groups = std::min<long long>(1024, std::max<long long>(1, P / 64));
     const long long rows = (P + groups - 1) / groups;
-    hipLaunchKernelGGL(bias_grad, dim3((unsigned)((P + rows - 1) / rows)), dim3(256), 0, N->ctx->stream,
-                       N->dz, P, C, rows, db);
+    hipLaunchKernelGGL(relu_bias_grad, dim3((unsigned)((P + rows - 1) / rows)), dim3(256), 0, N->ctx->stream,
+                       dY, lddy, dyoff, Y, ldy, yoff, N->dz, P, C, relu ? 1 : 0, rows, db);
 }
 
 // forward 3x3 conv of X (channel count cin, all channels) into Y (ldy, channel offset yoff):
@@ -169,9 +171,7 @@ static void conv_bwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, co
                      int yoff, const float* dY, int lddy, int dyoff, float* dX) {
     hipStream_t s = N->ctx->stream;
     const long long P = (long long)N->batch * nx * ny;
-    hipLaunchKernelGGL(relu_grad, dim3(nblk(P * L.cout)), dim3(256), 0, s, dY, lddy, dyoff, Y, ldy, yoff,
-                       N->dz, P, L.cout, L.relu ? 1 : 0);
-    bias_grad_launch(N, P, L.cout, N->G + L.b_off);
+    relu_bias_grad_launch(N, dY, lddy, dyoff, Y, ldy, yoff, P, L.cout, L.relu, N->G + L.b_off);
     // dW[co][tap*cin + ci] += sum_p dz[p][co] X[p + shift(tap)][ci]  (B gathers X)
     gemm(N, kKM, kKNconv, kAtomicF32, N->dz, L.cout, X, L.cin, N->G + L.w_off, 9 * L.cin, 0, L.cout,
          9 * L.cin, (int)P, nullptr, false, kWgradSlice, conv_geom(nx, ny, L.cin));
@@ -230,14 +230,11 @@ static void forward(bolt_nn* N) {
 static void backward(bolt_nn* N) {
     const int D = N->depth;
     auto& C = N->conv;
-    hipStream_t s = N->ctx->stream;
     const long long P0 = (long long)N->batch * N->nx * N->ny;
     // output 1x1 conv
     const Conv& O = C[N->lout()];
     const int w0 = O.cin;
-    hipLaunchKernelGGL(relu_grad, dim3(nblk(P0 * kOutC)), dim3(256), 0, s, N->dout, kOutC, 0, (const u16*)nullptr,
-                       0, 0, N->dz, P0, kOutC, 0);
-    bias_grad_launch(N, P0, kOutC, N->G + O.b_off);
+    relu_bias_grad_launch(N, N->dout, kOutC, 0, nullptr, 0, 0, P0, kOutC, false, N->G + O.b_off);
     gemm(N, kKM, kKN, kAtomicF32, N->dz, kOutC, N->u[0], w0, N->G + O.w_off, w0, 0, kOutC, w0, (int)P0, nullptr,
          false, kWgradSlice);
     gemm(N, kMK, kKN, kStoreF32, N->dz, kOutC, N->Wb + O.w_off, w0, N->du[0], w0, 0, (int)P0, w0, kOutC, nullptr,

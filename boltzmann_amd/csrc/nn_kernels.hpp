@@ -268,34 +268,33 @@ __global__ void flip_weights(const u16* __restrict__ W, int cout, int cin, u16* 
 }
 
 // dZ = dY * (Y > 0) as bf16, dY float with channel stride ld / offset off
-__global__ void relu_grad(const float* __restrict__ dY, int ldy, int yoff, const u16* __restrict__ Y,
-                          int ldY, int Yoff, u16* __restrict__ dZ, long long P, int C, int relu) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P * C) return;
-    const long long p = t / C;
-    const int c = (int)(t % C);
-    float g = dY[p * ldy + yoff + c];
-    if (relu && !(bf16_float(Y[p * ldY + Yoff + c]) > 0.f)) g = 0.f;
-    dZ[t] = bf16_bits(g);
-}
-
-// dbias[c] += sum_p dZ[p][c] (C <= 256, 256 threads).  Each work-group sums rows
-// [p0, p0 + rows) with contiguous loads: with the stride S = (256 / C) * C, a multiple of
-// C, thread t only ever sees channel t % C; then one atomic add per channel and group.
-__global__ void bias_grad(const u16* __restrict__ dZ, long long P, int C, long long rows,
-                          float* __restrict__ db) {
-    const int S = (int)(blockDim.x / C) * C;
+// Backward of a layer's bias + ReLU, in one pass over its P x C pre-activation gradient:
+//   dZ[p][c] = bf16(dY[p][c] masked by Y[p][c] > 0 when relu)  and  dbias[c] += sum_p dZ[p][c].
+// C <= 256 and 256 threads: with R = 256 / C rows per pass, thread t handles channel t % C of
+// rows p0 + t / C, + R, + 2R, ... of its work-group's chunk [p0, p0 + rows), then the group
+// adds its channel sums with one atomic per channel.
+__global__ void relu_bias_grad(const float* __restrict__ dY, int ldy, int yoff, const u16* __restrict__ Y,
+                               int ldY, int Yoff, u16* __restrict__ dZ, long long P, int C, int relu,
+                               long long rows, float* __restrict__ db) {
+    const int R = (int)blockDim.x / C;
+    const int c = (int)threadIdx.x % C, r0 = (int)threadIdx.x / C;
     const long long p0 = (long long)blockIdx.x * rows;
     const long long p1 = p0 + rows < P ? p0 + rows : P;
     float s = 0.f;
-    if ((int)threadIdx.x < S)
-        for (long long e = p0 * C + threadIdx.x; e < p1 * C; e += S) s += bf16_float(dZ[e]);
+    if (r0 < R)
+        for (long long p = p0 + r0; p < p1; p += R) {
+            float g = dY[p * ldy + yoff + c];
+            if (relu && !(bf16_float(Y[p * ldY + Yoff + c]) > 0.f)) g = 0.f;
+            const u16 b = bf16_bits(g);
+            dZ[p * C + c] = b;
+            s += bf16_float(b);
+        }
     __shared__ float sh[256];
     sh[threadIdx.x] = s;
     __syncthreads();
     if ((int)threadIdx.x < C) {
         float t = 0.f;
-        for (int j = threadIdx.x; j < S; j += C) t += sh[j];
+        for (int j = threadIdx.x; j < R * C; j += C) t += sh[j];
         atomicAdd(&db[threadIdx.x], t);
     }
 }
