@@ -185,21 +185,27 @@ static void conv_bwd(bolt_nn* N, const Conv& L, const u16* X, int nx, int ny, co
     }
 }
 
+// pooling and upsampling, 8 channels per thread (every channel count, leading dimension and
+// channel offset here is a multiple of 16)
 static void pool(bolt_nn* N, const u16* X, int ldx, int xoff, u16* Y, int nx, int ny, int C) {
-    hipLaunchKernelGGL(avgpool2, dim3(nblk((long long)N->batch * (nx / 2) * (ny / 2) * C)), dim3(256), 0,
-                       N->ctx->stream, X, ldx, xoff, Y, N->batch, nx, ny, C);
+    const uint32_t n = (uint32_t)((long long)N->batch * (nx / 2) * (ny / 2) * (C / 8));
+    hipLaunchKernelGGL(avgpool2, dim3(nblk(n)), dim3(256), 0, N->ctx->stream, X, ldx, xoff, Y, n,
+                       grid8(nx / 2, ny / 2, C));
 }
 static void pool_bwd(bolt_nn* N, const float* dY, float* dX, int lddx, int dxoff, int nx, int ny, int C) {
-    hipLaunchKernelGGL(avgpool2_grad, dim3(nblk((long long)N->batch * nx * ny * C)), dim3(256), 0,
-                       N->ctx->stream, dY, dX, lddx, dxoff, N->batch, nx, ny, C);
+    const uint32_t n = (uint32_t)((long long)N->batch * nx * ny * (C / 8));
+    hipLaunchKernelGGL(avgpool2_grad, dim3(nblk(n)), dim3(256), 0, N->ctx->stream, dY, dX, lddx, dxoff, n,
+                       grid8(nx, ny, C));
 }
 static void up(bolt_nn* N, const u16* X, u16* Y, int ldy, int yoff, int nx, int ny, int C) {
-    hipLaunchKernelGGL(upsample2, dim3(nblk((long long)N->batch * nx * ny * C)), dim3(256), 0,
-                       N->ctx->stream, X, Y, ldy, yoff, N->batch, nx, ny, C);
+    const uint32_t n = (uint32_t)((long long)N->batch * nx * ny * (C / 8));
+    hipLaunchKernelGGL(upsample2, dim3(nblk(n)), dim3(256), 0, N->ctx->stream, X, Y, ldy, yoff, n,
+                       grid8(nx, ny, C));
 }
 static void up_bwd(bolt_nn* N, const float* dY, int ldy, int yoff, float* dX, int nx, int ny, int C) {
-    hipLaunchKernelGGL(upsample2_grad, dim3(nblk((long long)N->batch * (nx / 2) * (ny / 2) * C)), dim3(256),
-                       0, N->ctx->stream, dY, ldy, yoff, dX, N->batch, nx, ny, C);
+    const uint32_t n = (uint32_t)((long long)N->batch * (nx / 2) * (ny / 2) * (C / 8));
+    hipLaunchKernelGGL(upsample2_grad, dim3(nblk(n)), dim3(256), 0, N->ctx->stream, dY, ldy, yoff, dX, n,
+                       grid8(nx / 2, ny / 2, C));
 }
 
 static void forward(bolt_nn* N) {

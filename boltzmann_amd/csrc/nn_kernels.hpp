@@ -299,75 +299,119 @@ __global__ void relu_bias_grad(const float* __restrict__ dY, int ldy, int yoff, 
     }
 }
 
-// 2x2 average pooling (channels-last), ld/off select a channel slice of X
-__global__ void avgpool2(const u16* __restrict__ X, int ldx, int xoff, u16* __restrict__ Y, int nb,
-                         int nx, int ny, int C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ox = nx / 2, oy = ny / 2;
-    if (t >= (long long)nb * ox * oy * C) return;
-    const int c = (int)(t % C);
-    long long r = t / C;
-    const int y = (int)(r % oy);
-    r /= oy;
-    const int x = (int)(r % ox);
-    const int b = (int)(r / ox);
-    float s = 0.f;
+// The pooling and upsampling kernels walk a channels-last image 8 channels per thread:
+// thread t takes pixel p = t / (C/8) and channels [c0, c0 + 8), c0 = 8 (t % (C/8)), with
+// p = (b*nx + x)*ny + y.  C, the leading dimensions and the channel offsets are multiples
+// of 8, so each thread moves 16 bytes of bf16 or 32 of float per pixel; the thread count
+// (pixels x C/8 <= 2^26 x 32) fits in 32 bits.
+struct Grid8 {
+    int nx, ny, C;
+    Divisor dx, dy, dq;  // division by nx, ny and C / 8
+};
+
+inline Grid8 grid8(int nx, int ny, int C) {
+    return {nx, ny, C, make_divisor((uint32_t)nx), make_divisor((uint32_t)ny), make_divisor((uint32_t)(C / 8))};
+}
+
+struct Pix8 {
+    uint32_t p;
+    int b, x, y, c0;
+};
+
+__device__ inline Pix8 pix8(const Grid8& g, uint32_t t) {
+    Pix8 r;
+    r.p = divide(t, g.dq);
+    r.c0 = (int)(t - r.p * (uint32_t)(g.C / 8)) * 8;
+    const uint32_t bx = divide(r.p, g.dy), b = divide(bx, g.dx);
+    r.y = (int)(r.p - bx * (uint32_t)g.ny);
+    r.x = (int)(bx - b * (uint32_t)g.nx);
+    r.b = (int)b;
+    return r;
+}
+
+__device__ inline void load8f(const float* p, float (&v)[8]) {
+    const float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+__device__ inline void store8f(float* p, const float (&v)[8]) {
+    *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+    *(float4*)(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+// 2x2 average pooling (channels-last) into Y [b][nx/2][ny/2][C]; ld/off select a channel
+// slice of X.  g describes the pooled image (nx/2, ny/2, C).
+__global__ void avgpool2(const u16* __restrict__ X, int ldx, int xoff, u16* __restrict__ Y, uint32_t n,
+                         Grid8 g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Pix8 q = pix8(g, t);
+    const int nx = 2 * g.nx, ny = 2 * g.ny;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            s += bf16_float(X[((size_t)((size_t)b * nx + 2 * x + i) * ny + 2 * y + j) * ldx + xoff + c]);
-    Y[t] = bf16_bits(0.25f * s);
+        for (int j = 0; j < 2; ++j) {
+            const size_t fp = ((size_t)q.b * nx + 2 * q.x + i) * ny + 2 * q.y + j;
+            const uint4 v = *(const uint4*)(X + fp * ldx + xoff + q.c0);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s[e] += bf16_float((u16)half16(v, e));
+        }
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        w[k] = (uint32_t)bf16_bits(0.25f * s[2 * k]) | ((uint32_t)bf16_bits(0.25f * s[2 * k + 1]) << 16);
+    *(uint4*)(Y + (size_t)q.p * g.C + q.c0) = {w[0], w[1], w[2], w[3]};
 }
 
-// backward of avgpool2: dX[b, x, y, c] (+)= dY[b, x/2, y/2, c] / 4
-__global__ void avgpool2_grad(const float* __restrict__ dY, float* __restrict__ dX, int lddx,
-                              int dxoff, int nb, int nx, int ny, int C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nb * nx * ny * C) return;
-    const int c = (int)(t % C);
-    const long long p = t / C;
-    const int y = (int)(p % ny);
-    const long long bx = p / ny;
-    const int x = (int)(bx % nx);
-    const int b = (int)(bx / nx);
-    const float g = 0.25f * dY[((size_t)((size_t)b * (nx / 2) + x / 2) * (ny / 2) + y / 2) * C + c];
-    dX[(size_t)p * lddx + dxoff + c] += g;
+// backward of avgpool2: dX[b, x, y, c] (+)= dY[b, x/2, y/2, c] / 4.  g describes the
+// full-resolution image (nx, ny, C).
+__global__ void avgpool2_grad(const float* __restrict__ dY, float* __restrict__ dX, int lddx, int dxoff,
+                              uint32_t n, Grid8 g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Pix8 q = pix8(g, t);
+    const size_t cp = ((size_t)q.b * (g.nx / 2) + q.x / 2) * (g.ny / 2) + q.y / 2;
+    float v[8], d[8];
+    load8f(dY + cp * g.C + q.c0, v);
+    float* out = dX + (size_t)q.p * lddx + dxoff + q.c0;
+    load8f(out, d);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) d[e] += 0.25f * v[e];
+    store8f(out, d);
 }
 
-// nearest 2x upsampling into a channel slice: Y[b, x, y, off + c] = X[b, x/2, y/2, c]
-__global__ void upsample2(const u16* __restrict__ X, u16* __restrict__ Y, int ldy, int yoff, int nb,
-                          int nx, int ny, int C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nb * nx * ny * C) return;
-    const int c = (int)(t % C);
-    const long long p = t / C;
-    const int y = (int)(p % ny);
-    const long long bx = p / ny;
-    const int x = (int)(bx % nx);
-    const int b = (int)(bx / nx);
-    Y[(size_t)p * ldy + yoff + c] = X[((size_t)((size_t)b * (nx / 2) + x / 2) * (ny / 2) + y / 2) * C + c];
+// nearest 2x upsampling into a channel slice: Y[b, x, y, off + c] = X[b, x/2, y/2, c].  g
+// describes the upsampled image (nx, ny, C).
+__global__ void upsample2(const u16* __restrict__ X, u16* __restrict__ Y, int ldy, int yoff, uint32_t n,
+                          Grid8 g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Pix8 q = pix8(g, t);
+    const size_t cp = ((size_t)q.b * (g.nx / 2) + q.x / 2) * (g.ny / 2) + q.y / 2;
+    *(uint4*)(Y + (size_t)q.p * ldy + yoff + q.c0) = *(const uint4*)(X + cp * g.C + q.c0);
 }
 
-// backward of upsample2: dX[b, x, y, c] = sum of the four dY[.., off + c]
+// backward of upsample2: dX[b, x, y, c] = sum of the four dY[.., off + c].  g describes the
+// coarse image (nx/2, ny/2, C).
 __global__ void upsample2_grad(const float* __restrict__ dY, int ldy, int yoff, float* __restrict__ dX,
-                               int nb, int nx, int ny, int C) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ox = nx / 2, oy = ny / 2;
-    if (t >= (long long)nb * ox * oy * C) return;
-    const int c = (int)(t % C);
-    long long r = t / C;
-    const int y = (int)(r % oy);
-    r /= oy;
-    const int x = (int)(r % ox);
-    const int b = (int)(r / ox);
-    float s = 0.f;
+                               uint32_t n, Grid8 g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Pix8 q = pix8(g, t);
+    const int nx = 2 * g.nx, ny = 2 * g.ny;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            s += dY[((size_t)((size_t)b * nx + 2 * x + i) * ny + 2 * y + j) * ldy + yoff + c];
-    dX[t] = s;
+        for (int j = 0; j < 2; ++j) {
+            float v[8];
+            load8f(dY + (((size_t)q.b * nx + 2 * q.x + i) * ny + 2 * q.y + j) * ldy + yoff + q.c0, v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s[e] += v[e];
+        }
+    store8f(dX + (size_t)q.p * g.C + q.c0, s);
 }
 
 // mean-squared error over all elements; dP = 2 (P - T) / n; partial sums per block
