@@ -356,7 +356,7 @@ class Predictor:
         """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8]."""
         return self.net.predict(x) * self.std + self.mean
 
-    def predict_cases(self, cases, mirror: bool = True) -> np.ndarray:
+    def predict_cases(self, cases, mirror: bool = False) -> np.ndarray:
         """Targets for cases; with mirror, the mean of the prediction and the reflected
         prediction of the reflected case (exact symmetry, so only the network's error
         differs between the two)."""
