@@ -50,8 +50,9 @@ inline Divisor make_divisor(uint32_t d) {
     return {(uint32_t)((((1ull << 32) * ((1ull << s) - d)) / d) + 1), s};
 }
 
-__device__ inline uint32_t divide(uint32_t n, Divisor v) {
-    return (uint32_t)(((uint64_t)__umulhi(n, v.m) + n) >> v.s);
+__host__ __device__ inline uint32_t divide(uint32_t n, Divisor v) {
+    const uint64_t hi = ((uint64_t)n * v.m) >> 32;  // mulhi: v_mul_hi_u32 on the device
+    return (uint32_t)((hi + n) >> v.s);
 }
 
 // Geometry of an implicit 3x3 (padding 1) convolution operand: the element for pixel p and
