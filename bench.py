@@ -187,38 +187,47 @@ def rb_batch(steps: int = 50000):
             "nusselt_256x130_400k_steps": nu_check}
 
 
-def cylinder(steps_marks=(1000, 5000, 20000)):
+def cylinder_flow(nx=4096, ny=1024, r=64, u0=0.05):
+    """Flags and initial velocity of the channel: fixed-velocity inflow column, fixed-density
+    outflow column, walls top and bottom and the disc (x-1024)^2 + (y-512)^2 < r^2 (scaled
+    with the channel), written last (vortex_street.py:100-115)."""
+    from boltzmann_amd.core import CellFlags
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    flags = np.zeros((nx, ny), np.int32)
+    flags[0, :] = CellFlags.FIXED_FLUID
+    flags[-1, :] = CellFlags.FIXED_FLUID_DENSITY
+    flags[:, 0] = CellFlags.WALL
+    flags[:, -1] = CellFlags.WALL
+    flags[(X - nx // 4) ** 2 + (Y - ny // 2) ** 2 < r * r] = CellFlags.WALL
+    vel = np.zeros((nx, ny, 2), np.float32)
+    vel[..., 0] = u0
+    vel[flags == CellFlags.WALL] = 0.0
+    return flags, vel
+
+
+def cylinder(steps_marks=(1000, 5000, 20000), tau_minus=0.52):
     """Flow past a disc in a 4096 x 1024 channel, D2Q9 TRT, Re = 1000 on the diameter:
     walls top and bottom, bounce-back disc r = 64 at (1024, 512), fixed-velocity inflow
     u = 0.05, fixed-density outflow.  Runs f64, f32, f16s and bf16s storage and reports the
     relative max-norm of u against the f64 run at each mark, plus MLUPS.
 
-    tau- is 0.52, as in the reference's vortex-street example (vortex_street.py:102).  With
-    the magic number 1/4 instead (tau- = 13.5 at tau+ = 0.5192) this flow diverges between
-    3,000 and 5,000 steps in f64 and f32 alike."""
+    tau- is 0.52, as in the reference's vortex-street example (vortex_street.py:102).  The
+    magic number 1/4 (tau- = 13.52 at tau+ = 0.5192) makes this flow diverge: the record's
+    magic_quarter_f64 gives the first check (every 250 steps) at which the f64 run holds a
+    non-finite velocity."""
     import boltzmann_amd
     from boltzmann_amd import Omega
-    from boltzmann_amd.core import CellFlags
     nx, ny, r, u0 = 4096, 1024, 64, 0.05
     nu = u0 * 2 * r / 1000.0
-    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
-    disc = (X - 1024) ** 2 + (Y - 512) ** 2 < r * r
-    vel0 = np.zeros((nx, ny, 2), np.float32)
-    vel0[..., 0] = u0
-    out = {}
+    tau_p = 3 * nu + 0.5
+    flags, vel0 = cylinder_flow(nx, ny, r, u0)
+    out = {"tau_plus": round(tau_p, 6), "tau_minus": tau_minus}
     ref = {}
     for st in ("f64", "f32", "f16s", "bf16s"):
-        sim = boltzmann_amd.Simulation("gpu", [nx, ny], Omega.TRT(1 / (3 * nu + 0.5), 1 / 0.52),
+        sim = boltzmann_amd.Simulation("gpu", [nx, ny], Omega.TRT(1 / tau_p, 1 / tau_minus),
                                        storage=st)
-        flags = sim.cells.flags
-        flags[0, :] = CellFlags.FIXED_FLUID
-        flags[-1, :] = CellFlags.FIXED_FLUID_DENSITY
-        flags[:, 0] = CellFlags.WALL
-        flags[:, -1] = CellFlags.WALL
-        flags[disc] = CellFlags.WALL
-        v = vel0.copy()
-        v[flags == CellFlags.WALL] = 0.0
-        sim.fluid.vel[...] = v
+        sim.cells.flags[...] = flags
+        sim.fluid.vel[...] = vel0
         done, ms_tot, errs = 0, 0.0, {}
         sim.step(2)
         done = 2
@@ -233,6 +242,21 @@ def cylinder(steps_marks=(1000, 5000, 20000)):
         mlups = nx * ny * (done - 2) / (1e6 * ms_tot * 1e-3)
         out[st] = {"mlups": round(mlups, 1), **({"u_err_vs_f64": errs} if errs else {})}
         del sim
+    # the configuration's magic-number TRT: find where the f64 run stops being finite
+    tau_m = 0.25 / (tau_p - 0.5) + 0.5
+    sim = boltzmann_amd.Simulation("gpu", [nx, ny], Omega.TRT(1 / tau_p, 1 / tau_m), storage="f64")
+    sim.cells.flags[...] = flags
+    sim.fluid.vel[...] = vel0
+    sim.iterate(2)
+    bad_at = None
+    for step in range(250, 20001, 250):
+        sim.step(248 if step == 250 else 250)
+        if sim.count_nonfinite("vel"):
+            bad_at = step
+            break
+    out["magic_quarter_f64"] = {"tau_minus": round(tau_m, 4), "nonfinite_at_step": bad_at,
+                                "checked_to": step}
+    del sim
     return out
 
 

@@ -1237,7 +1237,7 @@ struct InstDesc {
     void* vel1;
     void* C;
     const uint8_t* flags;
-    uint32_t plane_bytes;
+    uint64_t plane_bytes;
     int nx, ny;
     int acc_mode;
     float om_p, om_n, om_g;
@@ -1308,13 +1308,21 @@ struct bolt_batch {
     std::vector<uint8_t> active;   // instances a step advances (empty: all)
 };
 
+// Destroys the captured graph.  The stream is drained first, so no replay of the graph is
+// still queued or running when its executable (and the kernel arguments it owns) goes away.
 static void batch_drop_graph(bolt_batch* B) {
-    if (B->graph) (void)hipGraphExecDestroy(B->graph);
+    if (B->graph) {
+        (void)hipStreamSynchronize(B->ctx->stream);
+        (void)hipGraphExecDestroy(B->graph);
+    }
     B->graph = nullptr;
     B->graph_block = B->graph_len = 0;
 }
 
 static int batch_build_tiles(bolt_batch* B, int block) {
+    // The captured graph holds the current tile table in its kernel arguments: it goes
+    // before the table does (and waits for the stream).
+    batch_drop_graph(B);
     std::vector<Tile> tiles;
     for (size_t k = 0; k < B->lat.size(); ++k) {
         if (!B->active.empty() && !B->active[k]) continue;
@@ -1335,7 +1343,6 @@ static int batch_build_tiles(bolt_batch* B, int block) {
     }
     B->ntiles = (int)tiles.size();
     B->block = block;
-    batch_drop_graph(B);
     return BOLT_OK;
 }
 
@@ -1375,7 +1382,7 @@ static int batch_enqueue(bolt_batch* B, uint64_t n) {
     hipStream_t s = B->ctx->stream;
     const uint64_t len = (uint64_t)B->graph_steps;
     uint64_t done = 0;
-    if (n > len + 2) {
+    if (len > 0 && n > len + 2) {  // graph_steps 0: every step is a direct launch
         if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len) {
             int rc = batch_capture(B, (int)len);
             if (rc) return rc;
@@ -1401,7 +1408,7 @@ static int batch_upload_desc(bolt_batch* B) {
         d.vel1 = (char*)L->vel + (size_t)L->ncell * 4;
         d.C = L->tracers.empty() ? nullptr : L->tracers[0].C;
         d.flags = L->flags;
-        d.plane_bytes = (uint32_t)(L->stride * 4);
+        d.plane_bytes = (uint64_t)L->stride * 4;
         d.nx = L->n[0];
         d.ny = L->n[1];
         d.om_p = L->om_p;
@@ -1441,9 +1448,10 @@ int bolt_batch_create(bolt_ctx* ctx, int n, const int64_t* counts, int collision
     B->collision = collision;
     B->tracers = omega_g != nullptr;
     for (int k = 0; k < n; ++k) {
-        if (counts[2 * k] > 65535) {
+        // tiles hold the instance, the row and the 64-cell chunk of the row in 16 bits each
+        if (counts[2 * k] > 65535 || counts[2 * k + 1] > 64ll * 65535) {
             for (auto* L : B->lat) bolt_lattice_destroy(L);
-            return fail(BOLT_EINVAL, "instance %d: nx above 65535", k);
+            return fail(BOLT_EINVAL, "instance %d: nx above 65535 or ny above 64*65535", k);
         }
         bolt_lattice* L = nullptr;
         int rc = bolt_lattice_create(ctx, 2, counts + 2 * k, 9, collision, omega_f[2 * k],
@@ -1558,8 +1566,8 @@ int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (block < 64 || block > 256 || block % 64 != 0)
         return fail(BOLT_EINVAL, "block must be 64, 128, 192 or 256");
-    if (graph_steps < 2 || graph_steps % 2 != 0 || graph_steps > 4096)
-        return fail(BOLT_EINVAL, "graph_steps must be even, 2..4096");
+    if (graph_steps < 0 || graph_steps % 2 != 0 || graph_steps > 4096)
+        return fail(BOLT_EINVAL, "graph_steps must be even, 0..4096");
     B->graph_steps = graph_steps;
     if (block != B->block) return batch_build_tiles(B, block);
     return BOLT_OK;

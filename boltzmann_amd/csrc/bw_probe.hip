@@ -12,6 +12,17 @@
 //   inplace27_h   the same as inplace27 with 2-byte elements (the f16s/bf16s storage)
 //   shift27_h     the same as shift27 with 2-byte elements
 //   inplace9      9 float planes in place over 3x the cells (the D2Q9 mix)
+//   inplace27_w2, inplace27_w4
+//                 inplace27 with 2 or 4 consecutive cells per lane (8- and 16-byte
+//                 aligned accesses, dwordx2 / dwordx4)
+//   shift27_w4    shift27 with 4 cells per lane: every access is an aligned 16-byte
+//                 load or store, a wavefront covers one 256-cell row, and the +-1 shift
+//                 along the row is a one-lane rotation across the wavefront (DPP)
+//   shift27_w4h   the same with the general-row handling a row longer than one
+//                 wavefront needs: lane 0 loads and stores the element before the
+//                 wavefront's range itself and the last lane stores three elements
+//   inplace27_hw4, shift27_hw4
+//                 inplace27_h and shift27_h with 4 cells per lane (8-byte accesses)
 // (each in-place figure the better of plain and streaming loads) and prints one JSON
 // object of GB/s (bytes read + written per second).  Every index is
 // checked against the element count it addresses.
@@ -87,6 +98,149 @@ __global__ void shifted_planes(T* __restrict__ f, long long n) {
     for (int q = 0; q < 27; ++q) f[a[q]] = v[(q + 1) % 27];
 }
 
+// ---- wide accesses: VW consecutive 4-byte words of a plane per lane ----------------------
+template <int VW>
+struct WideT;
+template <>
+struct WideT<2> {
+    typedef uint32_t T __attribute__((ext_vector_type(2)));
+};
+template <>
+struct WideT<4> {
+    typedef uint32_t T __attribute__((ext_vector_type(4)));
+};
+
+template <int Q, int VW, bool NT>
+__global__ void __launch_bounds__(256) inplace_wide(uint32_t* __restrict__ f, long long nv) {
+    using V = typename WideT<VW>::T;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;  // nv = vectors per plane
+    V* p = (V*)f;
+    V v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (NT) v[q] = __builtin_nontemporal_load(p + q * nv + t);
+        else v[q] = p[q * nv + t];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) p[q * nv + t] = v[(q + 1) % Q];
+}
+
+// One-lane rotations across the 64-lane wavefront (DPP wave_ror:1 / wave_rol:1):
+// ror1(x) on lane l returns x of lane (l - 1) mod 64, rol1(x) that of lane (l + 1) mod 64.
+__device__ inline uint32_t ror1(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x13C, 0xF, 0xF, false);
+}
+__device__ inline uint32_t rol1(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x134, 0xF, 0xF, false);
+}
+
+// Shift of one 4-word vector per lane by one element along the row, for 4-byte elements
+// (H = false) or 2-byte elements packed two to a word (H = true).
+//   down: element e of the result is element e - 1 of the row (lane 0 takes lane 63's last)
+//   up:   element e of the result is element e + 1 of the row (lane 63 takes lane 0's first)
+template <int VW, bool H>
+__device__ inline void shift_down(typename WideT<VW>::T& a) {
+    const uint32_t prev = ror1(a[VW - 1]);
+    if constexpr (!H) {
+#pragma unroll
+        for (int k = VW - 1; k > 0; --k) a[k] = a[k - 1];
+        a[0] = prev;
+    } else {
+#pragma unroll
+        for (int k = VW - 1; k > 0; --k) a[k] = __builtin_amdgcn_alignbyte(a[k], a[k - 1], 2);
+        a[0] = __builtin_amdgcn_alignbyte(a[0], prev, 2);
+    }
+}
+template <int VW, bool H>
+__device__ inline void shift_up(typename WideT<VW>::T& a) {
+    const uint32_t next = rol1(a[0]);
+    if constexpr (!H) {
+#pragma unroll
+        for (int k = 0; k < VW - 1; ++k) a[k] = a[k + 1];
+        a[VW - 1] = next;
+    } else {
+#pragma unroll
+        for (int k = 0; k < VW - 1; ++k) a[k] = __builtin_amdgcn_alignbyte(a[k + 1], a[k], 2);
+        a[VW - 1] = __builtin_amdgcn_alignbyte(next, a[VW - 1], 2);
+    }
+}
+
+// The D3Q27 neighbour mix with VW words per lane on a lattice of 256 x 256 rows of 64*VW
+// words each: one wavefront per row, row r = (block * 256 + thread) / 64.  Plane q's access
+// of row r is the aligned vector of the row at (x + dx, y + dy), shifted by dz elements
+// along the row by rotation.  HALO: the general handling a row longer than one wavefront
+// needs, where the element across the wavefront's edge belongs to another wavefront's
+// range.  For dz = -1 lane 0 loads and stores the element before the range itself and
+// lane 63 stores VW - 1 elements; for dz = +1 lane 63 loads and stores the element after
+// the range and lane 0 stores the last VW - 1.  (On these 256-cell rows that element is
+// the periodic wrap inside the same row.)  Every address is the plane base plus a row
+// index below 65536 times 64 plus a lane (vector units), or one word of those vectors.
+template <int VW, bool H, bool HALO>
+__global__ void __launch_bounds__(256) shift_wide(uint32_t* __restrict__ f, long long nv) {
+    using V = typename WideT<VW>::T;
+    static_assert(!(H && HALO), "the halo probe is for 4-byte elements");
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;  // nv = 65536 rows * 64 lanes
+    const int lane = (int)(t & 63);
+    const int r = (int)(t >> 6);
+    const int x = r >> 8, y = r & 255;
+    V* p = (V*)f;
+    V v[27];
+    long long a[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1, dy = (q / 3) % 3 - 1, dx = q / 9 - 1;
+        const int rr = (((x + dx) & 255) << 8) | ((y + dy) & 255);
+        a[q] = q * nv + ((long long)rr << 6) + lane;
+        v[q] = __builtin_nontemporal_load(p + a[q]);
+        if (dz == -1) {
+            shift_down<VW, H>(v[q]);
+            if (HALO && lane == 0)  // last word of the row: lane 63's vector
+                v[q][0] = __builtin_nontemporal_load((uint32_t*)(p + a[q] + 63) + VW - 1);
+        } else if (dz == 1) {
+            shift_up<VW, H>(v[q]);
+            if (HALO && lane == 63)  // first word of the row: lane 0's vector
+                v[q][VW - 1] = __builtin_nontemporal_load((uint32_t*)(p + a[q] - 63));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1;
+        V w = v[(q + 1) % 27];
+        if (dz == -1) {
+            const uint32_t edge = w[0];
+            shift_up<VW, H>(w);
+            if (HALO) {
+                uint32_t* pw = (uint32_t*)(p + a[q]);
+                if (lane == 0) *((uint32_t*)(p + a[q] + 63) + VW - 1) = edge;
+                if (lane == 63) {
+#pragma unroll
+                    for (int k = 0; k < VW - 1; ++k) pw[k] = w[k];
+                } else {
+                    p[a[q]] = w;
+                }
+                continue;
+            }
+        } else if (dz == 1) {
+            const uint32_t edge = w[VW - 1];
+            shift_down<VW, H>(w);
+            if (HALO) {
+                uint32_t* pw = (uint32_t*)(p + a[q]);
+                if (lane == 63) *(uint32_t*)(p + a[q] - 63) = edge;
+                if (lane == 0) {
+#pragma unroll
+                    for (int k = 1; k < VW; ++k) pw[k] = w[k];
+                } else {
+                    p[a[q]] = w;
+                }
+                continue;
+            }
+        }
+        p[a[q]] = w;
+    }
+}
+
 template <typename F>
 static double gbs(F launch, double bytes, int reps) {
     hipEvent_t a, b;
@@ -144,12 +298,34 @@ int main(int argc, char** argv) {
         gbs([&] { inplace_planes<9, float, false><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps),
         gbs([&] { inplace_planes<9, float, true><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps));
 
+    // wide accesses (vectors of VW words per lane; the 2-byte lattice is half the words)
+    const long long nw = n;  // words per float plane
+    double r_w2 = best(
+        gbs([&] { inplace_wide<27, 2, false><<<grid(nw / 2), b>>>((uint32_t*)f, nw / 2); }, lat, reps),
+        gbs([&] { inplace_wide<27, 2, true><<<grid(nw / 2), b>>>((uint32_t*)f, nw / 2); }, lat, reps));
+    double r_w4 = best(
+        gbs([&] { inplace_wide<27, 4, false><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); }, lat, reps),
+        gbs([&] { inplace_wide<27, 4, true><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); }, lat, reps));
+    double r_sw4 = gbs([&] { shift_wide<4, false, false><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); },
+                       lat, reps);
+    double r_sw4h = gbs([&] { shift_wide<4, false, true><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); },
+                        lat, reps);
+    // 2-byte elements, 4 per lane: the 256-cell rows are 128 words, 2 words per lane
+    double r_hw4 = best(
+        gbs([&] { inplace_wide<27, 2, false><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); }, half, reps),
+        gbs([&] { inplace_wide<27, 2, true><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); }, half, reps));
+    double r_shw4 = gbs([&] { shift_wide<2, true, false><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); },
+                        half, reps);
+
     hipDeviceProp_t p;
     CK(hipGetDeviceProperties(&p, 0));
     printf("{\"device\": \"%s\", \"bytes_per_pass\": %.0f, \"copy4\": %.1f, \"read27\": %.1f, "
            "\"write27\": %.1f, \"inplace27\": %.1f, \"shift27\": %.1f, \"inplace27_h\": %.1f, "
-           "\"shift27_h\": %.1f, \"inplace9\": %.1f}\n",
-           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_shh, r_in9);
+           "\"shift27_h\": %.1f, \"inplace9\": %.1f, \"inplace27_w2\": %.1f, "
+           "\"inplace27_w4\": %.1f, \"shift27_w4\": %.1f, \"shift27_w4h\": %.1f, "
+           "\"inplace27_hw4\": %.1f, \"shift27_hw4\": %.1f}\n",
+           p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_shh, r_in9, r_w2, r_w4,
+           r_sw4, r_sw4h, r_hw4, r_shw4);
     CK(hipFree(f));
     CK(hipFree(sink));
     return 0;

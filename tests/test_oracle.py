@@ -222,6 +222,36 @@ def test_f64_path_matches_textbook_tightly():
     np.testing.assert_allclose(s.f[m], ref.F[m], rtol=0, atol=1e-13)
 
 
+@pytest.mark.parametrize("counts,q", [((16, 12), 5), ((8, 6, 7), 7)])
+def test_f64_tracer_matches_textbook_tightly(counts, q):
+    # orc_update_ad_f64 on its own pin: a moving fluid (|u| ~ 0.05), fixed-scalar cells and
+    # walls, against the float64 push/pull textbook update
+    d = len(counts)
+    nsq = 9 if d == 2 else 27
+    rng = np.random.default_rng(7)
+    flags, rho, vel = _setup(counts, rng, nsq)
+    flags[rng.random(counts) < 0.05] |= 8
+    om = ("bgk", 1 / 0.6)
+    s = oracle.OracleSimulation(counts, om, q=nsq, dtype=np.float64)
+    s.flags[...] = flags
+    s.rho[...] = rho
+    s.vel[...] = vel
+    Cv = rng.random(counts)
+    tr = s.add_tracer("T", 1 / 0.8, q)
+    tr.C[...] = Cv
+    ref = tb.TextbookSim(counts, nsq, om, flags, rho, vel)
+    ref.add_tracer(q, 1 / 0.8, Cv)
+    assert np.abs(vel).max() > 0.02
+    for _ in range(3):
+        s.iterate(2)
+        ref.step()
+        ref.step()
+        m = tb.reachable_mask(flags, d, q)
+        np.testing.assert_allclose(tr.g[m], ref.tracers[0][2][m], rtol=0, atol=1e-13)
+        nw = (flags & 1) == 0
+        np.testing.assert_allclose(tr.C[nw], ref.tracers[0][3][nw], rtol=0, atol=1e-13)
+
+
 # ---- the reference's own integration asserts (python/tests/test_core.py:57-127) -----
 
 def test_reference_rest_state():  # test_core.py:57-79
