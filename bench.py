@@ -73,9 +73,10 @@ def omega(coll: str, tau: float = 0.6):
     return core.bgk(tau) if coll == "bgk" else core.trt(tau)
 
 
-def run_gpu(n, coll, storage, steps, warmup, rho, vel, keep_vel=False):
+def run_gpu(n, coll, storage, steps, warmup, rho, vel, keep_vel=False, vw=0):
     import boltzmann_amd
     sim = boltzmann_amd.Simulation("gpu", [n, n, n], omega(coll), storage=storage)
+    sim.set_vector_width(vw)
     sim.fluid.rho[...] = rho
     sim.fluid.vel[...] = vel
     sim.step(2)  # first-iterate initialisation + upload, then 2 steps
@@ -376,6 +377,8 @@ def main() -> int:
     ap.add_argument("--ws-epochs", type=int, default=20)
     ap.add_argument("--nn-throughput", action="store_true",
                     help="add the warm-start network's training / inference throughput")
+    ap.add_argument("--vw-sweep", action="store_true",
+                    help="add MLUPS of every storage format at 1, 2 and 4 cells per lane")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -422,6 +425,15 @@ def main() -> int:
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
+    if args.vw_sweep:
+        sweep = {}
+        for st in ("f32", "f16s", "bf16s", "f32f", "f64"):
+            for vw in ((1, 2) if st == "f64" else (1, 2, 4)):
+                r = run_gpu(n, "trt", st, steps, args.warmup, rho, vel, vw=vw)
+                sweep[f"{st}_vw{vw}"] = round(r["mlups"], 1)
+                print(f"[bench] trt {st} {vw} cells per lane: {r['mlups']:.0f} MLUPS",
+                      file=sys.stderr, flush=True)
+        variants["vector_width_sweep_mlups"] = sweep
     if args.nn_throughput:
         variants["nn_unet_512x256"] = nn_throughput()
         print(f"[bench] network: {variants['nn_unet_512x256']}", file=sys.stderr, flush=True)

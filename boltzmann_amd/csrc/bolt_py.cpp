@@ -631,6 +631,10 @@ PYBIND11_MODULE(_boltzmann, m) {
             return out;
         }, py::arg("what"), py::arg("origin") = py::none())
         .def("set_block", [](PySimulation& s, int b) { check(bolt_lattice_set_block(s.core->lat, b), "set_block"); })
+        .def("set_vector_width", [](PySimulation& s, int vw) {
+            // cells per lane of the fluid step (0: chosen per lattice); results do not change
+            check(bolt_lattice_set_vector_width(s.core->lat, vw), "set_vector_width");
+        }, py::arg("vw"))
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;
             check(bolt_lattice_device_bytes(s.core->lat, &b), "device_bytes");

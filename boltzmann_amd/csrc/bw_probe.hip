@@ -23,6 +23,9 @@
 //                 wavefront's range itself and the last lane stores three elements
 //   inplace27_hw4, shift27_hw4
 //                 inplace27_h and shift27_h with 4 cells per lane (8-byte accesses)
+//   shift27_w2, shift27_w2h
+//                 shift27_w4 and shift27_w4h with 2 cells per lane (8-byte accesses) on
+//                 rows of 128 cells
 // (each in-place figure the better of plain and streaming loads) and prints one JSON
 // object of GB/s (bytes read + written per second).  Every index is
 // checked against the element count it addresses.
@@ -166,24 +169,25 @@ __device__ inline void shift_up(typename WideT<VW>::T& a) {
     }
 }
 
-// The D3Q27 neighbour mix with VW words per lane on a lattice of 256 x 256 rows of 64*VW
-// words each: one wavefront per row, row r = (block * 256 + thread) / 64.  Plane q's access
+// The D3Q27 neighbour mix with VW words per lane on a lattice of (rows / 256) x 256 rows of
+// 64*VW words each: one wavefront per row, row r = (block * 256 + thread) / 64.  Plane q's access
 // of row r is the aligned vector of the row at (x + dx, y + dy), shifted by dz elements
 // along the row by rotation.  HALO: the general handling a row longer than one wavefront
 // needs, where the element across the wavefront's edge belongs to another wavefront's
 // range.  For dz = -1 lane 0 loads and stores the element before the range itself and
 // lane 63 stores VW - 1 elements; for dz = +1 lane 63 loads and stores the element after
-// the range and lane 0 stores the last VW - 1.  (On these 256-cell rows that element is
-// the periodic wrap inside the same row.)  Every address is the plane base plus a row
-// index below 65536 times 64 plus a lane (vector units), or one word of those vectors.
+// the range and lane 0 stores the last VW - 1.  (On these one-wavefront rows that element
+// is the periodic wrap inside the same row.)  Every address is the plane base plus a row
+// index below nv / 64 times 64 plus a lane (vector units), or one word of those vectors.
 template <int VW, bool H, bool HALO>
 __global__ void __launch_bounds__(256) shift_wide(uint32_t* __restrict__ f, long long nv) {
     using V = typename WideT<VW>::T;
     static_assert(!(H && HALO), "the halo probe is for 4-byte elements");
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nv) return;  // nv = 65536 rows * 64 lanes
+    if (t >= nv) return;  // nv = rows * 64 lanes, rows = 256 * (a power of two)
     const int lane = (int)(t & 63);
     const int r = (int)(t >> 6);
+    const int xmask = (int)(nv >> 14) - 1;  // rows / 256 - 1
     const int x = r >> 8, y = r & 255;
     V* p = (V*)f;
     V v[27];
@@ -191,7 +195,7 @@ __global__ void __launch_bounds__(256) shift_wide(uint32_t* __restrict__ f, long
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         const int dz = q % 3 - 1, dy = (q / 3) % 3 - 1, dx = q / 9 - 1;
-        const int rr = (((x + dx) & 255) << 8) | ((y + dy) & 255);
+        const int rr = (((x + dx) & xmask) << 8) | ((y + dy) & 255);
         a[q] = q * nv + ((long long)rr << 6) + lane;
         v[q] = __builtin_nontemporal_load(p + a[q]);
         if (dz == -1) {
@@ -310,6 +314,11 @@ int main(int argc, char** argv) {
                        lat, reps);
     double r_sw4h = gbs([&] { shift_wide<4, false, true><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); },
                         lat, reps);
+    // 2 cells per lane: rows of 128 cells, 512 x 256 of them
+    double r_sw2 = gbs([&] { shift_wide<2, false, false><<<grid(nw / 2), b>>>((uint32_t*)f, nw / 2); },
+                       lat, reps);
+    double r_sw2h = gbs([&] { shift_wide<2, false, true><<<grid(nw / 2), b>>>((uint32_t*)f, nw / 2); },
+                        lat, reps);
     // 2-byte elements, 4 per lane: the 256-cell rows are 128 words, 2 words per lane
     double r_hw4 = best(
         gbs([&] { inplace_wide<27, 2, false><<<grid(nw / 4), b>>>((uint32_t*)f, nw / 4); }, half, reps),
@@ -323,9 +332,10 @@ int main(int argc, char** argv) {
            "\"write27\": %.1f, \"inplace27\": %.1f, \"shift27\": %.1f, \"inplace27_h\": %.1f, "
            "\"shift27_h\": %.1f, \"inplace9\": %.1f, \"inplace27_w2\": %.1f, "
            "\"inplace27_w4\": %.1f, \"shift27_w4\": %.1f, \"shift27_w4h\": %.1f, "
-           "\"inplace27_hw4\": %.1f, \"shift27_hw4\": %.1f}\n",
+           "\"inplace27_hw4\": %.1f, \"shift27_hw4\": %.1f, \"shift27_w2\": %.1f, "
+           "\"shift27_w2h\": %.1f}\n",
            p.gcnArchName, lat, r_copy, r_read, r_write, r_in, r_sh, r_inh, r_shh, r_in9, r_w2, r_w4,
-           r_sw4, r_sw4h, r_hw4, r_shw4);
+           r_sw4, r_sw4h, r_hw4, r_shw4, r_sw2, r_sw2h);
     CK(hipFree(f));
     CK(hipFree(sink));
     return 0;

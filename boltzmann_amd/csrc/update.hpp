@@ -254,37 +254,43 @@ __device__ inline void ref_bgk(float (&f)[V::q], const float (&u)[3], float vv, 
     }
 }
 
-// The reference-arithmetic collision (lib.cl:266-274 BGK, 701-719 TRT: f+/- halves,
-// omega+ on the even part, omega- on the odd part, double polynomials narrowed to float)
-// fused with the AA stores, one opposite pair (I, opp I) at a time, so each population's
-// register is free as soon as it is stored.
+// The reference-arithmetic collision of one opposite pair (I, opp I) of one cell
+// (lib.cl:266-274 BGK, 701-719 TRT: f+/- halves, omega+ on the even part, omega- on the odd
+// part, double polynomials narrowed to float).
+template <class V, int COLL, int I>
+__device__ inline void ref_pair(float (&f)[V::q], const float (&u)[3], float vv, float r,
+                                float omp, float omn) {
+#pragma clang fp contract(off)
+    constexpr int J = V::opp(I);
+    if constexpr (COLL == 0) {
+        {
+            const double pI = ref_feq_poly<V, I>(u, vv);
+            f[I] = (float)((double)f[I] + (double)omp * ((double)r * pI - (double)f[I]));
+        }
+        if constexpr (J != I) {
+            const double pJ = ref_feq_poly<V, J>(u, vv);
+            f[J] = (float)((double)f[J] + (double)omp * ((double)r * pJ - (double)f[J]));
+        }
+    } else {
+        const double pp = ref_feq_pos<V, I>(u, vv);
+        const double pn = ref_feq_neg<V, I>(u);
+        const float fpos = (float)((double)omp * ((double)r * pp - (double)(f[I] + f[J]) * 0.5));
+        const float fneg = (float)((double)omn * ((double)r * pn - (double)(f[I] - f[J]) * 0.5));
+        f[I] = f[I] + (fpos + fneg);
+        if constexpr (J != I) f[J] = f[J] + (fpos - fneg);
+    }
+}
+
+// The reference-arithmetic collision fused with the AA stores, one opposite pair at a time,
+// so each population's register is free as soon as it is stored.
 template <class V, bool EVEN, int COLL, class NB, int I = 0>
 __device__ inline void ref_collide_store(float (&f)[V::q], const float (&u)[3], float vv, float r,
                                          float omp, float omn, void* const* planes, const NB& nb,
                                          uint32_t cell) {
-#pragma clang fp contract(off)
     if constexpr (I < V::q) {
         constexpr int J = V::opp(I);
         if constexpr (J >= I) {
-            if constexpr (COLL == 0) {
-                {
-                    const double p = ref_feq_poly<V, I>(u, vv);
-                    f[I] = (float)((double)f[I] + (double)omp * ((double)r * p - (double)f[I]));
-                }
-                if constexpr (J != I) {
-                    const double p = ref_feq_poly<V, J>(u, vv);
-                    f[J] = (float)((double)f[J] + (double)omp * ((double)r * p - (double)f[J]));
-                }
-            } else {
-                const double pp = ref_feq_pos<V, I>(u, vv);
-                const double pn = ref_feq_neg<V, I>(u);
-                const float fpos =
-                    (float)((double)omp * ((double)r * pp - (double)(f[I] + f[J]) * 0.5));
-                const float fneg =
-                    (float)((double)omn * ((double)r * pn - (double)(f[I] - f[J]) * 0.5));
-                f[I] = f[I] + (fpos + fneg);
-                if constexpr (J != I) f[J] = f[J] + (fpos - fneg);
-            }
+            ref_pair<V, COLL, I>(f, u, vv, r, omp, omn);
             if constexpr (EVEN) {
                 pop_store<float>(planes[I], nb.at(-V::c(I, 0), -V::c(I, 1), -V::c(I, 2)), f[J]);
                 if constexpr (J != I)
@@ -342,9 +348,12 @@ __device__ inline float cdotu(int i, const float (&u)[3]) {
 // ---- float arithmetic on deviations d_i = f_i - w_i (kF32F and the shifted formats) ----
 // rho - 1 = sum d, j = sum c d (the w_i cancel pairwise), and the update
 //   d_i += omega (w_i (rho - 1) + w_i rho (3 cu + 9/2 cu^2 - 3/2 u.u) - d_i)
-// never forms w_i + d_i, which keeps the small deviations exact to float rounding.
+// never forms w_i + d_i, which keeps the small deviations exact to float rounding.  The
+// multiply-adds are written out (fmaf) and nothing else is contracted, so every kernel that
+// uses these functions produces the same bits.
 template <class V>
 __device__ inline void dev_moments(const float (&d)[V::q], float& dr, float (&u)[3]) {
+#pragma clang fp contract(off)
     dr = 0.f;
     float j[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -359,43 +368,69 @@ __device__ inline void dev_moments(const float (&d)[V::q], float& dr, float (&u)
     for (int a = 0; a < V::d; ++a) u[a] = j[a] * inv;
 }
 
-template <class V>
-__device__ inline void dev_bgk(float (&d)[V::q], const float (&u)[3], float vv, float dr,
-                               float om) {
-    const float r = 1.0f + dr;
-    const float m15vv = -1.5f * vv;
-#pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        const float cu = cdotu<V>(i, u);
-        const float deq = V::w(i) * (dr + r * (m15vv + cu * (3.0f + 4.5f * cu)));
-        d[i] += om * (deq - d[i]);
+__device__ inline float dev_vv(const float (&u)[3]) {
+#pragma clang fp contract(off)
+    return u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+}
+
+// BGK of slot I
+template <class V, int I>
+__device__ inline void dev_bgk_slot(float (&d)[V::q], const float (&u)[3], float m15vv, float dr,
+                                    float r, float om) {
+#pragma clang fp contract(off)
+    const float cu = cdotu<V>(I, u);
+    const float t = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), m15vv);
+    const float deq = V::w(I) * __builtin_fmaf(r, t, dr);
+    d[I] = __builtin_fmaf(om, deq - d[I], d[I]);
+}
+
+// TRT of the pair (I, opp I) (I odd), or of the rest slot (I = 0)
+template <class V, int I>
+__device__ inline void dev_trt_pair(float (&d)[V::q], const float (&u)[3], float m15vv, float dr,
+                                    float r, float omp, float omn) {
+#pragma clang fp contract(off)
+    if constexpr (I == 0) {
+        const float eq0 = V::w(0) * __builtin_fmaf(r, m15vv, dr);
+        d[0] = __builtin_fmaf(omp, eq0 - d[0], d[0]);
+    } else {
+        constexpr int J = V::opp(I);
+        const float cu = cdotu<V>(I, u);
+        const float eqp = V::w(I) * __builtin_fmaf(r, __builtin_fmaf(4.5f * cu, cu, m15vv), dr);
+        const float eqm = V::w(I) * (r * 3.0f * cu);
+        const float dp = omp * (eqp - 0.5f * (d[I] + d[J]));
+        const float dm = omn * (eqm - 0.5f * (d[I] - d[J]));
+        d[I] += dp + dm;
+        d[J] += dp - dm;
     }
 }
 
-template <class V>
-__device__ inline void dev_trt(float (&d)[V::q], const float (&u)[3], float vv, float dr,
-                               float omp, float omn) {
-    const float r = 1.0f + dr;
-    const float m15vv = -1.5f * vv;
-    d[0] += omp * (V::w(0) * (dr + r * m15vv) - d[0]);
-#pragma unroll
-    for (int i = 1; i < V::q; i += 2) {
-        const int j = i + 1;
-        const float cu = cdotu<V>(i, u);
-        const float eqp = V::w(i) * (dr + r * (m15vv + 4.5f * cu * cu));
-        const float eqm = V::w(i) * (r * 3.0f * cu);
-        const float dp = omp * (eqp - 0.5f * (d[i] + d[j]));
-        const float dm = omn * (eqm - 0.5f * (d[i] - d[j]));
-        d[i] += dp + dm;
-        d[j] += dp - dm;
+// the collision of the opposite pair (I, opp I), I <= opp I, of one cell
+template <class V, int COLL, int I>
+__device__ inline void dev_pair(float (&d)[V::q], const float (&u)[3], float m15vv, float dr,
+                                float r, float omp, float omn) {
+    if constexpr (COLL == 0) {
+        dev_bgk_slot<V, I>(d, u, m15vv, dr, r, omp);
+        if constexpr (V::opp(I) != I) dev_bgk_slot<V, V::opp(I)>(d, u, m15vv, dr, r, omp);
+    } else {
+        dev_trt_pair<V, I>(d, u, m15vv, dr, r, omp, omn);
+    }
+}
+
+template <class V, int COLL, int I = 0>
+__device__ inline void dev_collide(float (&d)[V::q], const float (&u)[3], float vv, float dr,
+                                   float omp, float omn) {
+    if constexpr (I < V::q) {
+        if constexpr (V::opp(I) >= I) {
+#pragma clang fp contract(off)
+            dev_pair<V, COLL, I>(d, u, -1.5f * vv, dr, 1.0f + dr, omp, omn);
+        }
+        dev_collide<V, COLL, I + 1>(d, u, vv, dr, omp, omn);
     }
 }
 
 // ---- double arithmetic (kF64) -------------------------------------------------------------
 template <class V>
-__device__ inline void f64_step(double (&f)[V::q], double& r, double (&u)[3], bool fixed,
-                                int coll, const StepParams& p, double rfix,
-                                const double (&ufix)[3], const double (&acc)[3], bool use_acc) {
+__device__ inline void f64_moments(const double (&f)[V::q], double& r, double (&u)[3]) {
 #pragma clang fp contract(off)
     r = 0.0;
     u[0] = u[1] = u[2] = 0.0;
@@ -407,7 +442,55 @@ __device__ inline void f64_step(double (&f)[V::q], double& r, double (&u)[3], bo
     }
 #pragma unroll
     for (int a = 0; a < V::d; ++a) u[a] /= r;
-    double op = p.om_p, on = p.om_n;
+}
+
+template <class V, int I>
+__device__ inline double f64_feq(double r, const double (&u)[3], double vv) {
+#pragma clang fp contract(off)
+    double cu = 0.0;
+#pragma unroll
+    for (int a = 0; a < V::d; ++a) cu += (double)V::c(I, a) * u[a];
+    const double w = (double)V::wn(I) / (double)V::wd(I);
+    return w * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * vv);
+}
+
+template <class V, int COLL, int I>
+__device__ inline void f64_pair(double (&f)[V::q], double r, const double (&u)[3], double vv,
+                                double op, double on) {
+#pragma clang fp contract(off)
+    constexpr int J = V::opp(I);
+    const double ei = f64_feq<V, I>(r, u, vv);
+    const double ej = f64_feq<V, J>(r, u, vv);
+    if constexpr (COLL == 0) {
+        f[I] += op * (ei - f[I]);
+        if constexpr (J != I) f[J] += op * (ej - f[J]);
+    } else {
+        const double fpos = op * (0.5 * (ei + ej) - 0.5 * (f[I] + f[J]));
+        const double fneg = on * (0.5 * (ei - ej) - 0.5 * (f[I] - f[J]));
+        f[I] += fpos + fneg;
+        if constexpr (J != I) f[J] += fpos - fneg;
+    }
+}
+
+template <class V, int COLL, int I = 0>
+__device__ inline void f64_collide(double (&f)[V::q], double r, const double (&u)[3], double vv,
+                                   double op, double on) {
+    if constexpr (I < V::q) {
+        if constexpr (V::opp(I) >= I) f64_pair<V, COLL, I>(f, r, u, vv, op, on);
+        f64_collide<V, COLL, I + 1>(f, r, u, vv, op, on);
+    }
+}
+
+// moments, forcing, fixed-cell override and vv of one cell in double
+template <class V>
+__device__ inline void f64_prepare(const double (&f)[V::q], double& r, double (&u)[3], double& vv,
+                                   double& op, double& on, bool fixed, const StepParams& p,
+                                   double rfix, const double (&ufix)[3], const double (&acc)[3],
+                                   bool use_acc) {
+#pragma clang fp contract(off)
+    f64_moments<V>(f, r, u);
+    op = p.om_p;
+    on = p.om_n;
     if (use_acc)
 #pragma unroll
         for (int a = 0; a < V::d; ++a) u[a] += acc[a] / op;
@@ -418,32 +501,18 @@ __device__ inline void f64_step(double (&f)[V::q], double& r, double (&u)[3], bo
 #pragma unroll
         for (int a = 0; a < V::d; ++a) u[a] = ufix[a];
     }
-    double vv = 0.0;
+    vv = 0.0;
 #pragma unroll
     for (int a = 0; a < V::d; ++a) vv += u[a] * u[a];
-    double feq[V::q];
-#pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        double cu = 0.0;
-#pragma unroll
-        for (int a = 0; a < V::d; ++a) cu += (double)V::c(i, a) * u[a];
-        const double w = (double)V::wn(i) / (double)V::wd(i);
-        feq[i] = w * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * vv);
-    }
-    if (coll == 0) {
-#pragma unroll
-        for (int i = 0; i < V::q; ++i) f[i] += op * (feq[i] - f[i]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < V::q; ++i) {
-            const int j = V::opp(i);
-            if (j < i) continue;
-            const double fpos = op * (0.5 * (feq[i] + feq[j]) - 0.5 * (f[i] + f[j]));
-            const double fneg = on * (0.5 * (feq[i] - feq[j]) - 0.5 * (f[i] - f[j]));
-            f[i] += fpos + fneg;
-            if (j != i) f[j] += fpos - fneg;
-        }
-    }
+}
+
+template <class V, int COLL>
+__device__ inline void f64_step(double (&f)[V::q], double& r, double (&u)[3], bool fixed,
+                                const StepParams& p, double rfix, const double (&ufix)[3],
+                                const double (&acc)[3], bool use_acc) {
+    double vv, op, on;
+    f64_prepare<V>(f, r, u, vv, op, on, fixed, p, rfix, ufix, acc, use_acc);
+    f64_collide<V, COLL>(f, r, u, vv, op, on);
 }
 
 template <class V>
@@ -552,7 +621,7 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
             for (int a = 0; a < D; ++a) acc[a] = (double)p.g[a] * b;
         }
         double r, u[3];
-        f64_step<V>(f, r, u, fixed, COLL, p, rfix, ufix, acc, use_acc);
+        f64_step<V, COLL>(f, r, u, fixed, p, rfix, ufix, acc, use_acc);
         aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
         if (p.write_macro && !fixed) {
             elem<double>(p.rho, cell) = r;
@@ -617,17 +686,13 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
             const float vv = ref_vv<D>(u);
             ref_collide_store<V, EVEN, COLL>(f, u, vv, r, omp, omn, p.fpl, nb, cell);
         } else if constexpr (S == kF32F) {
-            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
-            else dev_trt<V>(f, u, vv, r, omp, omn);
+            dev_collide<V, COLL>(f, u, dev_vv(u), r, omp, omn);
 #pragma unroll
             for (int i = 0; i < Q; ++i) f[i] += V::w(i);
             aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
             r = r + 1.0f;
         } else {
-            const float vv = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            if constexpr (COLL == 0) dev_bgk<V>(f, u, vv, r, omp);
-            else dev_trt<V>(f, u, vv, r, omp, omn);
+            dev_collide<V, COLL>(f, u, dev_vv(u), r, omp, omn);
             T raw[Q];
 #pragma unroll
             for (int i = 0; i < Q; ++i) {

@@ -2,9 +2,11 @@
 vortex-street example (python/examples/vortex_street/vortex_street.py:100-115) through the
 product API, with fixed-velocity inflow, fixed-density outflow, bounce-back walls and disc.
 
-* A reduced channel (256 x 64, disc r = 8) is BIT-EXACT with the oracle in FP32 storage for
-  300 steps, both with the magic-number TRT (tau- = 1/4 / (tau+ - 1/2) + 1/2, the default
-  of core.py:40-62) and with tau- = 0.52 (the example's choice, vortex_street.py:100-102).
+* A reduced channel (256 x 64, disc r = 8) with the full configuration's relaxation rates
+  (tau+ = 0.5192 from u = 0.05, r = 64, Re = 1000) is BIT-EXACT with the oracle in FP32
+  storage for 300 steps, both with the magic-number TRT (tau- = 1/4 / (tau+ - 1/2) + 1/2,
+  the default of core.py:40-62) and with tau- = 0.52 (the example's choice,
+  vortex_street.py:100-102).
 * The full 4096 x 1024 channel (disc r = 64 at (1024, 512), u = 0.05, Re = 1000 on the
   diameter) is run for 1000 steps in f64, f32, f16s and bf16s storage: every format stays
   finite, its velocity is within a stated relative max-norm of the f64 run and its total
@@ -61,7 +63,7 @@ def test_reduced_cylinder_bit_exact(tau_minus):
     import boltzmann_amd
     from boltzmann_amd import core
     nx, ny, r, u0 = 256, 64, 8, 0.05
-    omp, omn = omegas(u0, r, 1000.0, tau_minus)
+    omp, omn = omegas(u0, 64, 1000.0, tau_minus)  # the 4096 x 1024 channel's rates
     flags, vel = channel(nx, ny, 64, 32, r, u0)
     sim = boltzmann_amd.Simulation("gpu", [nx, ny], core.Omega.TRT(omp, omn))
     ref = oracle.OracleSimulation((nx, ny), ("trt", omp, omn), q=9)
@@ -83,8 +85,13 @@ def test_reduced_cylinder_bit_exact(tau_minus):
 
 
 # relative max-norm of u against the f64 run after 1000 steps, and relative distance of the
-# total mass, per storage format (measured: f32 ~3e-5, f16s ~3e-3, bf16s ~3e-2 in u)
-BOUNDS = {"f32": (1e-4, 1e-6), "f16s": (1e-2, 1e-4), "bf16s": (6e-2, 1e-3)}
+# total mass, per tau- and storage format.  Measured with tau- = 0.52: f32 ~3e-5, f16s ~3e-3,
+# bf16s ~3e-2 in u; the magic number's slowly relaxed odd moments (tau- = 13.5) carry the
+# rounding further: f16s ~1.5e-2.
+BOUNDS = {
+    0.52: {"f32": (1e-4, 1e-6), "f16s": (1e-2, 1e-4), "bf16s": (6e-2, 1e-3)},
+    "magic": {"f32": (1e-4, 1e-6), "f16s": (3e-2, 1e-4), "bf16s": (2e-1, 1e-3)},
+}
 
 
 @pytest.mark.parametrize("tau_minus", ["magic", 0.52])
@@ -107,8 +114,9 @@ def test_full_cylinder_formats_against_f64(tau_minus):
     u64, m64 = res["f64"]
     assert np.isfinite(u64).all()
     assert np.abs(u64[..., 1]).max() > 1e-3  # the flow round the disc has developed
-    for st, (ub, mb) in BOUNDS.items():
-        u, m = res[st]
-        err = np.abs(u - u64).max() / np.abs(u64).max()
-        assert err < ub, (st, err)
-        assert abs(m - m64) / m64 < mb, (st, m, m64)
+    errs = {st: (float(np.abs(res[st][0] - u64).max() / np.abs(u64).max()),
+                 abs(res[st][1] - m64) / m64) for st in BOUNDS[tau_minus]}
+    print(f"tau- {tau_minus}: u error and mass distance vs f64 after 1000 steps: {errs}")
+    for st, (ub, mb) in BOUNDS[tau_minus].items():
+        assert errs[st][0] < ub, (st, errs)
+        assert errs[st][1] < mb, (st, errs)
