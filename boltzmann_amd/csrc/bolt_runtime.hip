@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/bolt.h"
 #include "update.hpp"
@@ -492,6 +493,8 @@ const char* bolt_last_error(void) { return g_last_error.c_str(); }
 
 const char* bolt_version(void) { return "bolt-hip 0.1 gfx950"; }
 
+static int scratch_setup(int device);
+
 int bolt_ctx_create(int device_kind, int device_index, bolt_ctx** out) {
     if (!out) return fail(BOLT_EINVAL, "out is null");
     *out = nullptr;
@@ -511,6 +514,11 @@ int bolt_ctx_create(int device_kind, int device_index, bolt_ctx** out) {
         return fail(BOLT_ENOTFOUND, "device %d is %s; this library is built for gfx950",
                     device_index, prop.gcnArchName);
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int rc = scratch_setup(device_index);  // the read-back staging area on this GPU
+    if (rc) {
+        (void)hipStreamDestroy(c->stream);
+        return rc;
+    }
     c->info = std::string("AMD GPU ") + prop.name + " (" + prop.gcnArchName + ")";
     *out = c.release();
     return BOLT_OK;
@@ -725,9 +733,12 @@ static void launch_pop_to_host(bolt_lattice* L, H* dst, long long c0, long long 
 
 static size_t host_size(int dt) { return dt == BOLT_DT_F64 ? 8 : 4; }
 
-// Moves a field in chunks of cells through a device staging buffer.
+// Moves a field in chunks of cells through a device staging buffer: one it allocates
+// (64 MiB at most) and frees, or, when `fixed_stage` is given, that buffer of
+// `fixed_stage_bytes` (nothing is allocated then).
 static int transfer(bolt_lattice* L, int field, int tracer_id, int host_dtype, void* host,
-                    size_t nbytes, bool to_device) {
+                    size_t nbytes, bool to_device, void* fixed_stage = nullptr,
+                    size_t fixed_stage_bytes = 0) {
     if (!L || !host) return fail(BOLT_EINVAL, "null argument");
     FieldDesc fd;
     int rc = describe(L, field, tracer_id, fd);
@@ -741,11 +752,12 @@ static int transfer(bolt_lattice* L, int field, int tracer_id, int host_dtype, v
     if (nbytes != want)
         return fail(BOLT_EINVAL, "field %d needs %zu bytes, got %zu", field, want, nbytes);
     HIP_TRY(hipSetDevice(L->ctx->device));
-    const size_t stage_bytes = 64u << 20;
+    const size_t stage_bytes = fixed_stage ? fixed_stage_bytes : (64u << 20);
     long long chunk = (long long)(stage_bytes / (fd.per_cell * es));
+    if (chunk < 1) return fail(BOLT_EINVAL, "staging area of %zu bytes holds no cell", stage_bytes);
     if (chunk > L->ncell) chunk = L->ncell;
-    void* stage = nullptr;
-    HIP_TRY(hipMalloc(&stage, (size_t)chunk * fd.per_cell * es));
+    void* stage = fixed_stage;
+    if (!stage) HIP_TRY(hipMalloc(&stage, (size_t)chunk * fd.per_cell * es));
     hipStream_t s = L->ctx->stream;
     const bool f64m = is_f64(L);
     int status = BOLT_OK;
@@ -798,7 +810,7 @@ static int transfer(bolt_lattice* L, int field, int tracer_id, int host_dtype, v
             status = fail(BOLT_EDEVICE, "field transfer failed: %s", hipGetErrorString(e));
     }
     (void)hipStreamSynchronize(s);
-    (void)hipFree(stage);
+    if (!fixed_stage) (void)hipFree(stage);
     return status;
 }
 
@@ -810,6 +822,44 @@ int bolt_lattice_write(bolt_lattice* L, int field, int tracer_id, int host_dtype
 int bolt_lattice_read(bolt_lattice* L, int field, int tracer_id, int host_dtype, void* host,
                       size_t nbytes) {
     return transfer(L, field, tracer_id, host_dtype, host, nbytes, false);
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free read-back.  `scratch` is the staging area of bolt_lattice_read_noalloc:
+// every GPU the library runs on holds its own copy for the life of the process.  In host
+// code the array's name is only a stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device when a context is created on it
+// (bolt_ctx_create) and kept in g_scratch_dev.  All calls on one GPU share that GPU's
+// array: calls that overlap on one GPU (two threads, two contexts) race on it.
+// ---------------------------------------------------------------------------------------
+static constexpr size_t kNoAllocScratchBytes = 16u << 20;  // 16 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Resolves the device address of `scratch` on `device`, which must be the current device.
+static int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(BOLT_EINVAL, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return BOLT_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(BOLT_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_dev[device] = ptr;
+    return BOLT_OK;
+}
+
+int bolt_lattice_read_noalloc(bolt_lattice* L, int field, int tracer_id, int host_dtype,
+                              void* host, size_t nbytes) {
+    if (!L || !host) return fail(BOLT_EINVAL, "null argument");
+    const int dev = L->ctx->device;
+    void* ptr = (dev >= 0 && dev < kMaxDevices) ? g_scratch_dev[dev] : nullptr;
+    if (!ptr) return fail(BOLT_ESTATE, "no staging area on device %d (contexts set it up)", dev);
+    return transfer(L, field, tracer_id, host_dtype, host, nbytes, false, ptr, kNoAllocScratchBytes);
 }
 
 int bolt_lattice_equilibrate(bolt_lattice* L) {

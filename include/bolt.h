@@ -123,6 +123,17 @@ int bolt_lattice_write(bolt_lattice* lat, int field, int tracer_id, int host_dty
 int bolt_lattice_read(bolt_lattice* lat, int field, int tracer_id, int host_dtype, void* host,
                       size_t nbytes);
 
+/* bolt_lattice_read without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- tight serving loops that read the
+ * fields back after every step call, code that must not call hipMalloc / hipFree (which
+ * bolt_lattice_read does for its 64 MiB staging buffer on every call).  The field is
+ * staged through a fixed 16 MiB area that each GPU holds for the life of the process;
+ * larger fields move through it in pieces.  All calls on one GPU share that area: calls
+ * that overlap on one GPU (several threads or contexts) race and must be serialised by
+ * the caller.  Same arguments, checks and result as bolt_lattice_read. */
+int bolt_lattice_read_noalloc(bolt_lattice* lat, int field, int tracer_id, int host_dtype,
+                              void* host, size_t nbytes);
+
 /* Device-side equilibrium initialisation: f = feq(rho, vel) and, for every tracer,
  * g = geq(C, vel), in float with the reference's host formula (velocities.rs:43-52). */
 int bolt_lattice_equilibrate(bolt_lattice* lat);
