@@ -339,6 +339,52 @@ def nn_throughput(nx: int = 512, ny: int = 256, batch: int = 8, width: int = 32,
             "predict_samples_per_s": round(infer, 1)}
 
 
+def profile_kernels(steps: int = 20) -> dict:
+    """A short run of every hot-path kernel for a profiler: the 256^3 D3Q27 TRT lattice in
+    every storage format, the 4096 x 1024 D2Q9 TRT cylinder channel in f32 and f16s, a
+    1024 x 1024 D2Q9 lattice with a fused D2Q5 tracer and Boussinesq coupling, and the
+    64-cell Rayleigh-Benard batch with direct launches (no graph replay).  Returns the
+    launches and the algorithmic bytes per launch of each workload."""
+    import boltzmann_amd
+    from boltzmann_amd import Omega, core, rb
+    out = {}
+    n = 256
+    rho, vel = tgv(n)
+    for st in ("f32", "f16s", "bf16s", "f64"):
+        run_gpu(n, "trt", st, steps, 0, rho, vel)
+        out[f"tgv256_d3q27_trt_{st}"] = {"launches": steps + 2, "bytes_per_launch": n ** 3 * B_ALG[st]}
+    del rho, vel
+    flags, vel0 = cylinder_flow()
+    nu = 0.05 * 128 / 1000.0
+    for st, b in (("f32", 73), ("f16s", 37)):
+        sim = boltzmann_amd.Simulation("gpu", [4096, 1024], Omega.TRT(1 / (3 * nu + 0.5), 1 / 0.52),
+                                       storage=st)
+        sim.cells.flags[...] = flags
+        sim.fluid.vel[...] = vel0
+        sim.step(2)
+        sim.step_timed(10 * steps)
+        out[f"cylinder_4096x1024_d2q9_trt_{st}"] = {"launches": 10 * steps + 2,
+                                                    "bytes_per_launch": 4096 * 1024 * b}
+        del sim
+    m = 1024
+    sim = boltzmann_amd.Simulation("gpu", [m, m], Omega.TRT(1 / 0.6, 1 / 0.9))
+    rng = np.random.default_rng(0)
+    t = sim.add_tracer("T", core.bgk(0.6))
+    t.val[...] = rng.random((m, m)).astype(np.float32)
+    sim.add_boussinesq_coupling(t, -1e-3, 0.5, np.array([0.0, -1.0], np.float32))
+    sim.step(2)
+    sim.step_timed(10 * steps)
+    out["fused_1024x1024_d2q9_trt_d2q5"] = {"launches": 10 * steps + 2, "bytes_per_launch": m * m * 121}
+    del sim
+    b = rb.make_batch(rb.sweep(64, seed=4))
+    b.set_shape(128, 0)
+    b.step(10 * steps)
+    b.step_timed(2)
+    out["rb_batch_64"] = {"launches": 10 * steps + 2, "bytes_per_launch": int(b.cells) * 121}
+    del b
+    return out
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of nbytes (hipMemcpy): a kernel of known traffic for
     scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -379,6 +425,9 @@ def main() -> int:
                     help="add the warm-start network's training / inference throughput")
     ap.add_argument("--vw-sweep", action="store_true",
                     help="add MLUPS of every storage format at 1, 2 and 4 cells per lane")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only run every hot-path kernel briefly (for rocprofv3) and print "
+                         "the algorithmic bytes per launch of each workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice update here is built and measured on a "
@@ -387,6 +436,11 @@ def main() -> int:
         return 2
     steps = args.steps + args.steps % 2
     n = args.n
+    if args.profile_kernels:
+        if args.copy_calibration:
+            copy_calibration()
+        print(json.dumps({"profile_kernels": profile_kernels(min(steps, 20))}), flush=True)
+        return 0
     ceilings = None if args.no_probe else memory_ceilings()
     if ceilings is not None:
         print(f"[bench] memory ceilings: {ceilings}", file=sys.stderr, flush=True)

@@ -87,10 +87,10 @@ def test_reduced_cylinder_bit_exact(tau_minus):
 # relative max-norm of u against the f64 run after 1000 steps, and relative distance of the
 # total mass, per tau- and storage format.  Measured with tau- = 0.52: f32 ~3e-5, f16s ~3e-3,
 # bf16s ~3e-2 in u; the magic number's slowly relaxed odd moments (tau- = 13.5) carry the
-# rounding further: f16s ~1.5e-2.
+# rounding further: f16s ~1.5e-2, bf16s ~2e-1.
 BOUNDS = {
     0.52: {"f32": (1e-4, 1e-6), "f16s": (1e-2, 1e-4), "bf16s": (6e-2, 1e-3)},
-    "magic": {"f32": (1e-4, 1e-6), "f16s": (3e-2, 1e-4), "bf16s": (2e-1, 1e-3)},
+    "magic": {"f32": (1e-4, 1e-6), "f16s": (3e-2, 1e-4), "bf16s": (3e-1, 1e-3)},
 }
 
 
