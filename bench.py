@@ -39,17 +39,38 @@ B_ALG = {"f32": 2 * 27 * 4 + 1, "f32f": 2 * 27 * 4 + 1, "f16s": 2 * 27 * 2 + 1,
          "bf16s": 2 * 27 * 2 + 1, "f64": 2 * 27 * 8 + 1}
 
 
+COUNTERS = os.path.join("profiles", "hot_path_kernels_hbm_counters.md")
+
+
+def kernel_sources_sha() -> str:
+    """sha256 prefix of the headline kernel's sources (update.hpp then lattice.hpp)."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("update.hpp", "lattice.hpp"):
+        with open(os.path.join(ROOT, "boltzmann_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic():
     """HBM bytes per launch of the FP32 D3Q27 update from the committed counter summary
-    (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, calibrated on a 1 GiB copy)."""
-    path = os.path.join(ROOT, "profiles", "lbm_step_d3q27_f32_tgv256_hbm_counters.md")
+    (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, calibrated on 1 GiB copies),
+    with the file it came from; the bytes are None when the kernel's sources changed since
+    those counters were taken."""
+    vals = {}
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+        for line in open(os.path.join(ROOT, COUNTERS)):
+            if ":" in line and not line.startswith((" ", "|", "#")):
+                k, _, v = line.partition(":")
+                vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None, {"source": COUNTERS, "status": "missing"}
+    info = {"source": COUNTERS, "kernel_sources_sha256_16": vals.get("kernel_sources_sha256_16")}
+    if vals.get("kernel_sources_sha256_16") != kernel_sources_sha():
+        info["status"] = "stale: the kernel sources changed since these counters were taken"
+        return None, info
+    info["status"] = "current"
+    return int(vals["traffic_bytes_per_launch_d3q27_f32"]), info
 
 
 def tgv(n: int, u0: float = 0.04):
@@ -300,6 +321,33 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
             "held_out": s}
 
 
+def warmstart_small(n_train: int = 600, n_test: int = 32, epochs: int = 60):
+    """The learned warm start at the reduced size of its GPU test, bounded to a few minutes:
+    128 x 64 channels (obstacle size 4-10, Re 10-40), n_train solver-generated cases, a
+    depth-5 network without and with the coarse-lattice prior (the steady state on a 2x
+    coarser lattice, upsampled, given as input; the network learns the correction), each
+    compared with the cold start on n_test held-out cases to the 1e-6 residual.  Each warm
+    run decides its own fallback against a cold twin in the same batch (both twins'
+    steps charged); time_to_residual_ratio counts features, coarse solve and inference."""
+    from boltzmann_amd import warmstart as ws
+    kw = dict(nx=128, ny=64, re_range=(10.0, 40.0), size_range=(4, 10))
+    t0 = time.perf_counter()
+    cases, fields, _ = ws.generate(n_train, seed0=0, batch=300, max_steps=40000, check_every=1000, **kw)
+    out = {"lattice": [128, 64], "train_cases": len(cases), "generate_s": round(time.perf_counter() - t0, 1)}
+    held = [ws.make_case(50_000 + s, **kw) for s in range(n_test)]
+    for name, pf in (("plain", 0), ("coarse_prior", 2)):
+        t0 = time.perf_counter()
+        model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=32, depth=5, lr=2e-3,
+                                 seed=0, prior_factor=pf)
+        t_train = time.perf_counter() - t0
+        s = ws.compare(model, held, check_every=200, max_steps=40000, judge_at=400).summary()
+        s["train_s"] = round(t_train, 1)
+        out[name] = s
+        print(f"[bench] warm start 128x64 {name}: {s}", file=sys.stderr, flush=True)
+        del model
+    return out
+
+
 def memory_ceilings():
     """GB/s of copy / read-only / write-only / in-place plane kernels moving the bytes of one
     256^3 D3Q27 step (boltzmann_amd/bw_probe, a child process run before this process uses
@@ -423,6 +471,8 @@ def main() -> int:
     ap.add_argument("--ws-epochs", type=int, default=20)
     ap.add_argument("--nn-throughput", action="store_true",
                     help="add the warm-start network's training / inference throughput")
+    ap.add_argument("--no-warmstart-small", action="store_true",
+                    help="skip the bounded 128x64 warm-start leg of the default run")
     ap.add_argument("--vw-sweep", action="store_true",
                     help="add MLUPS of every storage format at 1, 2 and 4 cells per lane")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -479,6 +529,8 @@ def main() -> int:
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
+        if not args.no_warmstart_small:
+            variants["warmstart_channel_128x64"] = warmstart_small()
     if args.vw_sweep:
         sweep = {}
         for st in ("f32", "f16s", "bf16s", "f32f", "f64"):
@@ -496,6 +548,7 @@ def main() -> int:
             args.ws_train, args.ws_test, args.ws_epochs)
 
     achieved = head["mlups"] * 1e6 * B_ALG["f32"] / 1e9  # GB/s
+    traffic, traffic_info = measured_traffic()
     line = {
         "metric": "MLUPS & % HBM roofline (D3Q27 FP32/FP16); NN warm-start vs plain time-to-steady",
         "value": round(head["mlups"], 1),
@@ -515,7 +568,8 @@ def main() -> int:
                    "storage": "f32"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic() if n == 256 else None,
+                     "traffic": traffic if n == 256 else None,
+                     "traffic_source": traffic_info,
                      "bytes_per_lup": B_ALG["f32"]},
         "wall_mlups": round(head["wall_mlups"], 1),
         "run_sim_mlups": run_sim_rate,

@@ -38,7 +38,7 @@ using namespace nnk;
 namespace {
 
 constexpr int kInC = 16;   // padded input channels
-constexpr int kFeat = 8;   // caller's feature channels
+constexpr int kFeat = 8;   // caller's feature channels unless bolt_nn_set_features changes it
 constexpr int kOutC = 3;
 
 inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -59,6 +59,7 @@ struct Buf {
 struct bolt_nn {
     bolt_ctx* ctx = nullptr;
     int nx = 0, ny = 0, batch = 0, w = 32, depth = 3;
+    int nfeat = kFeat;     // feature channels of the host inputs (<= kInC)
     std::vector<Conv> conv;
     std::vector<int> ch, dch;   // encoder / decoder channels per level (0..depth)
     size_t nparam = 0;
@@ -296,16 +297,16 @@ static int adam(bolt_nn* N, float lr) {
     return BOLT_OK;
 }
 
-// host features [n][nx][ny][8] (float) -> device [n][nx][ny][16] bf16 (zero padded)
+// host features [n][nx][ny][nfeat] (float) -> device [n][nx][ny][16] bf16 (zero padded)
 // Uploads the float features of `cnt` samples and pads them to bf16 [pixels][kInC] at dst
 // on the device; pixels of the samples after `cnt` (up to `slots` samples) are zeroed.
 static int upload_features(bolt_nn* N, const float* x, int cnt, int slots, u16* dst) {
     const size_t pix = (size_t)N->nx * N->ny;
     hipStream_t s = N->ctx->stream;
-    HIP_TRY(hipMemcpyAsync(N->xf, x, (size_t)cnt * pix * kFeat * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(N->xf, x, (size_t)cnt * pix * N->nfeat * 4, hipMemcpyHostToDevice, s));
     const long long n = (long long)slots * pix;
-    hipLaunchKernelGGL((pad_features<kFeat, kInC>), dim3(nblk(n * kInC)), dim3(256), 0, s, N->xf, n,
-                       (long long)cnt * pix, dst);
+    hipLaunchKernelGGL((pad_features<kInC>), dim3(nblk(n * kInC)), dim3(256), 0, s, N->xf, n,
+                       (long long)cnt * pix, N->nfeat, dst);
     HIP_TRY(hipGetLastError());
     return BOLT_OK;
 }
@@ -392,7 +393,7 @@ int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int dept
     n->dpin.assign(D + 1, nullptr);
     n->du.assign(D, nullptr);
     A((void**)&n->x0, P0 * kInC * 2);
-    A((void**)&n->xf, P0 * kFeat * 4);
+    A((void**)&n->xf, P0 * kInC * 4);
     n->pin[0] = n->x0;
     for (int l = 0; l <= D; ++l) {
         const size_t P = P0 >> (2 * l);
@@ -469,6 +470,19 @@ int bolt_nn_set_params(bolt_nn* N, const float* in, uint64_t n) {
     return BOLT_OK;
 }
 
+int bolt_nn_set_features(bolt_nn* N, int nfeat) {
+    if (!N) return fail(BOLT_EINVAL, "null network");
+    if (nfeat < 1 || nfeat > kInC) return fail(BOLT_EINVAL, "feature channels must be 1..%d", kInC);
+    N->nfeat = nfeat;
+    return BOLT_OK;
+}
+
+int bolt_nn_features(const bolt_nn* N, int* out) {
+    if (!N || !out) return fail(BOLT_EINVAL, "null argument");
+    *out = N->nfeat;
+    return BOLT_OK;
+}
+
 int bolt_nn_loss_grad(bolt_nn* N, const float* x, const float* y, double* loss_out, float* grads, uint64_t ng) {
     if (!N || !x || !y) return fail(BOLT_EINVAL, "null argument");
     if (grads && ng != N->nparam) return fail(BOLT_EINVAL, "expected %zu gradients", N->nparam);
@@ -504,7 +518,7 @@ int bolt_nn_set_data(bolt_nn* N, int n, const float* x, const float* y) {
     // a batch of samples at a time through the float staging buffer (stream-ordered reuse)
     for (int i0 = 0; i0 < n; i0 += N->batch) {
         const int cnt = std::min(N->batch, n - i0);
-        int rc = upload_features(N, x + (size_t)i0 * pix * kFeat, cnt, cnt, N->data_x + (size_t)i0 * pix * kInC);
+        int rc = upload_features(N, x + (size_t)i0 * pix * N->nfeat, cnt, cnt, N->data_x + (size_t)i0 * pix * kInC);
         if (rc) return rc;
     }
     HIP_TRY(hipMemcpyAsync(N->data_y, y, (size_t)n * pix * kOutC * 4, hipMemcpyHostToDevice, N->ctx->stream));
@@ -570,7 +584,7 @@ int bolt_nn_predict(bolt_nn* N, int n, const float* x, float* y) {
     const size_t pix = (size_t)N->nx * N->ny;
     for (int i0 = 0; i0 < n; i0 += N->batch) {
         const int cnt = std::min(N->batch, n - i0);
-        int rc = upload_batch_inputs(N, x + (size_t)i0 * pix * kFeat, cnt, nullptr);
+        int rc = upload_batch_inputs(N, x + (size_t)i0 * pix * N->nfeat, cnt, nullptr);
         if (rc) return rc;
         forward(N);
         HIP_TRY(hipGetLastError());

@@ -254,7 +254,7 @@ struct PySimulation {
 struct NetCore {
     bolt_ctx* ctx = nullptr;
     bolt_nn* net = nullptr;
-    int nx = 0, ny = 0, batch = 0;
+    int nx = 0, ny = 0, batch = 0, nfeat = 8;
     ~NetCore() {
         if (net) bolt_nn_destroy(net);
         if (ctx) bolt_ctx_destroy(ctx);
@@ -696,6 +696,12 @@ PYBIND11_MODULE(_boltzmann, m) {
             check(bolt_batch_residual(b.core->batch, k, steps_between, &r), "residual");
             return r;
         }, py::arg("inst"), py::arg("steps_between"))
+        .def("residuals", [](PyBatch& b, uint64_t steps_between) {
+            // every active instance's residual in one launch (NaN: inactive)
+            py::array_t<double> out((ssize_t)b.core->shapes.size());
+            check(bolt_batch_residuals(b.core->batch, steps_between, out.mutable_data()), "residuals");
+            return out;
+        }, py::arg("steps_between"))
         .def_property_readonly("n", [](PyBatch& b) { return b.core->shapes.size(); })
         .def_property_readonly("cells", [](PyBatch& b) {
             uint64_t c = 0;
@@ -761,17 +767,20 @@ PYBIND11_MODULE(_boltzmann, m) {
     // ---- the warm-start network ----
     py::class_<PyNet>(m, "Network")
         .def(py::init([](const std::string& dev, int nx, int ny, int batch, int width, uint64_t seed,
-                         int depth) {
+                         int depth, int features) {
                  auto n = std::make_shared<NetCore>();
                  check(bolt_ctx_create(device_kind(dev), 0, &n->ctx), "device");
                  check(bolt_nn_create(n->ctx, nx, ny, batch, width, depth, seed, &n->net), "Network");
+                 check(bolt_nn_set_features(n->net, features), "Network");
                  n->nx = nx;
                  n->ny = ny;
                  n->batch = batch;
+                 n->nfeat = features;
                  return PyNet{n};
              }),
              py::arg("dev"), py::arg("nx"), py::arg("ny"), py::arg("batch"), py::arg("width") = 32,
-             py::arg("seed") = 0, py::arg("depth") = 3)
+             py::arg("seed") = 0, py::arg("depth") = 3, py::arg("features") = 8)
+        .def_property_readonly("features", [](PyNet& n) { return n.core->nfeat; })
         .def_property_readonly("param_count", [](PyNet& n) {
             uint64_t c = 0;
             check(bolt_nn_param_count(n.core->net, &c), "param_count");
@@ -791,8 +800,8 @@ PYBIND11_MODULE(_boltzmann, m) {
                              py::array_t<float, py::array::c_style | py::array::forcecast> y) {
             NetCore& c = *n.core;
             const size_t pix = (size_t)c.batch * c.nx * c.ny;
-            if ((size_t)x.size() != pix * 8 || (size_t)y.size() != pix * 3)
-                throw BoltError("loss_grad: x must be [batch, nx, ny, 8] and y [batch, nx, ny, 3]");
+            if ((size_t)x.size() != pix * c.nfeat || (size_t)y.size() != pix * 3)
+                throw BoltError("loss_grad: x must be [batch, nx, ny, features] and y [batch, nx, ny, 3]");
             uint64_t np_ = 0;
             check(bolt_nn_param_count(c.net, &np_), "param_count");
             py::array_t<float> g((ssize_t)np_);
@@ -804,9 +813,9 @@ PYBIND11_MODULE(_boltzmann, m) {
                             py::array_t<float, py::array::c_style | py::array::forcecast> y) {
             NetCore& c = *n.core;
             const size_t pix = (size_t)c.nx * c.ny;
-            const size_t cnt = (size_t)x.size() / (pix * 8);
-            if (cnt * pix * 8 != (size_t)x.size() || (size_t)y.size() != cnt * pix * 3)
-                throw BoltError("set_data: x must be [n, nx, ny, 8] and y [n, nx, ny, 3]");
+            const size_t cnt = (size_t)x.size() / (pix * c.nfeat);
+            if (cnt * pix * c.nfeat != (size_t)x.size() || (size_t)y.size() != cnt * pix * 3)
+                throw BoltError("set_data: x must be [n, nx, ny, features] and y [n, nx, ny, 3]");
             check(bolt_nn_set_data(c.net, (int)cnt, x.data(), y.data()), "set_data");
         })
         .def("train", [](PyNet& n, int epochs, float lr, uint64_t seed, int first, int total) {
@@ -820,8 +829,9 @@ PYBIND11_MODULE(_boltzmann, m) {
         .def("predict", [](PyNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> x) {
             NetCore& c = *n.core;
             const size_t pix = (size_t)c.nx * c.ny;
-            const size_t cnt = (size_t)x.size() / (pix * 8);
-            if (cnt * pix * 8 != (size_t)x.size() || cnt == 0) throw BoltError("predict: x must be [n, nx, ny, 8]");
+            const size_t cnt = (size_t)x.size() / (pix * c.nfeat);
+            if (cnt * pix * c.nfeat != (size_t)x.size() || cnt == 0)
+                throw BoltError("predict: x must be [n, nx, ny, features]");
             py::array_t<float> y(std::vector<ssize_t>{(ssize_t)cnt, c.nx, c.ny, 3});
             check(bolt_nn_predict(c.net, (int)cnt, x.data(), y.mutable_data()), "predict");
             return y;

@@ -1417,6 +1417,11 @@ struct bolt_batch {
     uint64_t cells = 0;
     bool tracers = true;
     std::vector<uint8_t> active;   // instances a step advances (empty: all)
+    // batched residual: per-instance descriptors, partial maxima and results on the device
+    struct ResDesc* d_res = nullptr;
+    double* d_res_part = nullptr;
+    double* d_res_out = nullptr;
+    std::vector<uint8_t> res_have; // instance holds a velocity snapshot
 };
 
 // Destroys the captured graph.  The stream is drained first, so no replay of the graph is
@@ -1598,6 +1603,9 @@ int bolt_batch_destroy(bolt_batch* B) {
     for (auto* L : B->lat) bolt_lattice_destroy(L);
     (void)hipFree(B->d_inst);
     (void)hipFree(B->d_tiles);
+    (void)hipFree(B->d_res);
+    (void)hipFree(B->d_res_part);
+    (void)hipFree(B->d_res_out);
     if (B->ev0) (void)hipEventDestroy(B->ev0);
     if (B->ev1) (void)hipEventDestroy(B->ev1);
     delete B;
@@ -1756,4 +1764,100 @@ int bolt_batch_residual(bolt_batch* B, int inst, uint64_t steps_between, double*
     bolt_lattice* L;
     int rc = batch_lat(B, inst, &L);
     return rc ? rc : bolt_lattice_residual(L, steps_between, out);
+}
+
+// The residual of every active instance in one launch: work-group (x, k) takes slice x of
+// instance k's cells (the arithmetic of k_residual), then one work-group per instance
+// reduces its slices.
+struct ResDesc {
+    const float* vel;     // D2Q9 instances store float velocity planes
+    double* snap;
+    const uint8_t* flags;
+    long long ncell;
+    int active, have;
+};
+
+constexpr int kResSlices = 16;
+
+__global__ void k_batch_residual(const ResDesc* desc, double* part) {
+    const ResDesc r = desc[blockIdx.y];
+    double dmax = 0.0, umax = 0.0;
+    if (r.active) {
+        for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < r.ncell;
+             c += (long long)gridDim.x * blockDim.x) {
+            double du2 = 0.0, u2 = 0.0;
+            for (int a = 0; a < 2; ++a) {
+                const double u = (double)r.vel[(long long)a * r.ncell + c];
+                const double w = u - r.snap[(long long)a * r.ncell + c];
+                du2 += w * w;
+                u2 += u * u;
+                r.snap[(long long)a * r.ncell + c] = u;
+            }
+            if (!(r.flags[c] & 1)) {
+                dmax = fmax(dmax, du2);
+                umax = fmax(umax, u2);
+            }
+        }
+    }
+    const double a = block_max(r.have ? dmax : 0.0);
+    const double b = block_max(umax);
+    if (threadIdx.x == 0) {
+        part[2 * ((long long)blockIdx.y * gridDim.x + blockIdx.x)] = a;
+        part[2 * ((long long)blockIdx.y * gridDim.x + blockIdx.x) + 1] = b;
+    }
+}
+
+__global__ void k_batch_residual_max(const double* part, int slices, double* out) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < slices; k += blockDim.x) {
+        a = fmax(a, part[2 * ((long long)blockIdx.x * slices + k)]);
+        b = fmax(b, part[2 * ((long long)blockIdx.x * slices + k) + 1]);
+    }
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sqrt(a);
+        out[2 * blockIdx.x + 1] = sqrt(b);
+    }
+}
+
+int bolt_batch_residuals(bolt_batch* B, uint64_t steps_between, double* out) {
+    if (!B || !out) return fail(BOLT_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(B->ctx->device));
+    hipStream_t s = B->ctx->stream;
+    const int n = (int)B->lat.size();
+    if (!B->d_res) {
+        for (auto* L : B->lat)
+            if (!L->vel_snap) {
+                int rc = dev_alloc(L, (void**)&L->vel_snap, (size_t)L->ncell * L->ndim * sizeof(double));
+                if (rc) return rc;
+            }
+        HIP_TRY(hipMalloc(&B->d_res, n * sizeof(ResDesc)));
+        HIP_TRY(hipMalloc(&B->d_res_part, (size_t)n * kResSlices * 2 * sizeof(double)));
+        HIP_TRY(hipMalloc(&B->d_res_out, (size_t)n * 2 * sizeof(double)));
+        B->res_have.assign(n, 0);
+    }
+    std::vector<ResDesc> h(n);
+    for (int k = 0; k < n; ++k) {
+        const bolt_lattice* L = B->lat[k];
+        h[k] = {(const float*)L->vel, L->vel_snap, L->flags, L->ncell,
+                B->active.empty() || B->active[k] ? 1 : 0, B->res_have[k]};
+    }
+    HIP_TRY(hipMemcpyAsync(B->d_res, h.data(), n * sizeof(ResDesc), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_batch_residual, dim3(kResSlices, n), dim3(256), 0, s, B->d_res, B->d_res_part);
+    hipLaunchKernelGGL(k_batch_residual_max, dim3(n), dim3(256), 0, s, B->d_res_part, kResSlices, B->d_res_out);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> r((size_t)n * 2);
+    HIP_TRY(hipMemcpyAsync(r.data(), B->d_res_out, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < n; ++k) {
+        if (!h[k].active) {
+            out[k] = NAN;
+            continue;
+        }
+        out[k] = (!h[k].have || steps_between == 0 || r[2 * k + 1] == 0.0)
+                     ? HUGE_VAL : r[2 * k] / ((double)steps_between * r[2 * k + 1]);
+        B->res_have[k] = 1;
+    }
+    return BOLT_OK;
 }

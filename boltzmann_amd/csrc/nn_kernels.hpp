@@ -453,14 +453,16 @@ __global__ void adam_step(float* __restrict__ w, u16* __restrict__ wb, const flo
 
 // Input features [pixels][FEAT] float -> [pixels][INC] bf16 with the channels from FEAT and
 // the pixels from `valid` on zero
-template <int FEAT, int INC>
+// features [pixels][feat] float -> [pixels][INC] bf16, channels feat.. and pixels valid..
+// zeroed (feat <= INC)
+template <int INC>
 __global__ void pad_features(const float* __restrict__ x, long long pixels, long long valid,
-                             u16* __restrict__ out) {
+                             int feat, u16* __restrict__ out) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pixels * INC) return;
     const long long p = t / INC;
     const int c = (int)(t - p * INC);
-    out[t] = (c < FEAT && p < valid) ? bf16_bits(x[p * FEAT + c]) : (u16)0;
+    out[t] = (c < feat && p < valid) ? bf16_bits(x[p * feat + c]) : (u16)0;
 }
 
 __global__ void to_bf16(const float* __restrict__ x, u16* __restrict__ y, long long n) {

@@ -161,6 +161,61 @@ def populations(case: Case, rho: np.ndarray, vel: np.ndarray) -> np.ndarray:
     return f.astype(np.float32)
 
 
+# ---- the coarse-lattice prior ------------------------------------------------------------
+def coarse_case(case: Case, factor: int = 2) -> Case:
+    """The same channel on a lattice `factor` times coarser: a coarse cell is a wall when at
+    least half of its fine cells are, the channel walls stay, u_in and Re are kept, so the
+    viscosity in coarse lattice units is nu / factor."""
+    nxc, nyc = case.nx // factor, case.ny // factor
+    m = case.mask[:nxc * factor, :nyc * factor].reshape(nxc, factor, nyc, factor).mean(axis=(1, 3)) >= 0.5
+    m[:, 0] = m[:, -1] = True
+    nu = (case.tau - 0.5) / 3.0 / factor
+    return Case(case.seed, nxc, nyc, case.u_in, case.re, 3 * nu + 0.5, m)
+
+
+def upsample(case: Case, factor: int, rho: np.ndarray, vel: np.ndarray):
+    """Bilinear interpolation of a coarse (rho, u) to the fine cell centres of `case`
+    (clamped at the lattice edges); u = 0 in the fine walls.  Non-finite coarse values (a
+    coarse run that blew up) give the cold start instead."""
+    if not (np.isfinite(rho).all() and np.isfinite(vel).all()):
+        return case.cold_state()
+    nxc, nyc = rho.shape
+
+    def axis(n_f, n_c):
+        t = (np.arange(n_f) + 0.5) / factor - 0.5
+        i0 = np.clip(np.floor(t).astype(np.int64), 0, n_c - 2)
+        return i0, np.clip(t - i0, 0.0, 1.0)
+
+    x0, tx = axis(case.nx, nxc)
+    y0, ty = axis(case.ny, nyc)
+
+    def up(a):
+        a = a.astype(np.float64)
+        TX, TY = tx[:, None], ty[None, :]
+        if a.ndim == 3:
+            TX, TY = TX[..., None], TY[..., None]
+        return ((1 - TX) * (1 - TY) * a[x0][:, y0] + TX * (1 - TY) * a[x0 + 1][:, y0]
+                + (1 - TX) * TY * a[x0][:, y0 + 1] + TX * TY * a[x0 + 1][:, y0 + 1])
+
+    r = up(rho).astype(np.float32)
+    v = up(vel).astype(np.float32)
+    v[case.mask] = 0.0
+    return r, v
+
+
+def coarse_prior(cases, factor: int = 2, tol: float = 1e-5, check_every: int = 500,
+                 max_steps: int = 40000, dev: str = "gpu"):
+    """Steady states of the cases on a `factor` times coarser lattice, upsampled to the
+    cases' lattice: [(rho, vel)] per case.  The plain solver runs them in one batch to tol
+    (a case that does not get there within max_steps, or stalls, gives its last state)."""
+    if not cases:
+        return []
+    cc = [coarse_case(c, factor) for c in cases]
+    _, fields = solve(cc, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
+                      drop_stalled=8)
+    return _each(lambda k: upsample(cases[k], factor, *fields[k]), range(len(cases)))
+
+
 @dataclass
 class SolveResult:
     steps: np.ndarray        # steps to reach tol (-1: not reached)
@@ -181,28 +236,37 @@ class SolveResult:
 
 def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="gpu",
           batch: Batch | None = None, stop_when_done=True, keep_fields=True, neq=False,
-          drop_stalled: int = 0):
+          drop_stalled: int = 0, judge_twins_at: int = 0):
     """Runs every case to the residual tolerance in one batch.
 
-    init: None for the cold start, else a list of (rho, vel) per case; with neq the
-    initial populations carry the non-equilibrium part of populations(), else they are at
-    equilibrium.  SolveResult.seconds covers the set-up (state upload, initial
-    populations) and the stepping.  With stop_when_done a converged case leaves the batch
-    (its state is kept); with drop_stalled > 0 so does a case whose residual has not halved
-    over that many checks (an unsteady flow; its steps stay -1).  Returns the SolveResult
-    and, with keep_fields, the final (rho, vel) of every case."""
+    init: None for the cold start, else a list of (rho, vel) per case (None entries start
+    cold); with neq the initial populations of the given states carry the non-equilibrium
+    part of populations(), else they are at equilibrium.  SolveResult.seconds covers the
+    set-up (state upload, initial populations) and the stepping.  With stop_when_done a
+    converged case leaves the batch (its state is kept); with drop_stalled > 0 so does a
+    case whose residual has not halved over that many checks (an unsteady flow; its steps
+    stay -1).
+
+    judge_twins_at > 0: the batch holds n cases twice, instances k (given start) and
+    n + k (its twin).  At the first check at or after that many steps, of each pair the
+    one with the larger residual leaves the batch (a converged one always stays);
+    SolveResult.rejected[k] is True where instance k lost to its twin and judged_at is the
+    step of that check.  Every step both twins ran is counted in case_steps.
+
+    Returns the SolveResult and, with keep_fields, the final (rho, vel) of every case."""
     t0 = time.perf_counter()
     b = batch or _batch(cases, dev)
     written = []
     for k, c in enumerate(cases):
-        if init is None:
+        if init is None or init[k] is None:
             written.append(load_state(b, k, c, *c.cold_state()))
         else:
             written.append(load_state(b, k, c, *init[k]))
     b.equilibrate()
     if neq and init is not None:
         for k, c in enumerate(cases):
-            b.write(k, "f", populations(c, *written[k]))
+            if init[k] is not None:
+                b.write(k, "f", populations(c, *written[k]))
     n = len(cases)
     steps = np.full(n, -1, np.int64)
     last = np.full(n, np.inf)
@@ -214,16 +278,19 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
     best_at = np.zeros(n, np.int64)
     checks = 0
     case_steps = 0
+    rejected = np.zeros(n, bool)
+    judged_at = 0
+    npair = n // 2 if judge_twins_at > 0 else 0
     while done < max_steps and active.any():
         case_steps += int(active.sum()) * check_every
         b.step(check_every - 100)
-        for k in np.flatnonzero(active):
-            b.residual(k, 0)          # snapshot u^(n-100)
+        b.residuals(0)                # snapshot u^(n-100) of every active case
         b.step(100)
         done += check_every
         checks += 1
+        rs = b.residuals(100)
         for k in np.flatnonzero(active):
-            r = b.residual(k, 100)
+            r = rs[k]
             last[k] = r
             if np.isnan(first[k]):
                 first[k] = r
@@ -233,6 +300,18 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
                 best[k], best_at[k] = r, checks
         hist.append(last.copy())
         leave = np.zeros(n, bool)
+        if npair and not judged_at and done >= judge_twins_at:
+            judged_at = done
+            for k in range(npair):
+                w, c = k, npair + k
+                if steps[w] >= 0:
+                    lose = c
+                elif steps[c] >= 0:
+                    lose = w
+                else:
+                    lose = w if last[w] > last[c] else c
+                rejected[lose] = True
+                leave[lose] = True
         if stop_when_done:
             leave |= steps >= 0
         if drop_stalled > 0:
@@ -242,6 +321,8 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
             b.set_active(active)
     secs = time.perf_counter() - t0
     res = SolveResult(steps, last, secs, first, np.array(hist), check_every, done, case_steps)
+    res.rejected = rejected
+    res.judged_at = judged_at
     if not keep_fields:
         return res, None
     fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
@@ -275,16 +356,19 @@ def _each(fn, items) -> list:
         return list(ex.map(fn, items))
 
 
-def features(cases) -> np.ndarray:
+def features(cases, prior=None) -> np.ndarray:
     """[n, nx, ny, 8] channels-last network inputs, per cell:
     0 wall mask, 1 u_in / 0.05, 2 (tau - 1/2) * 2, 3 Re / 100,
     4 x / nx (along the channel), 5 y / ny (across),
     6 chessboard distance to the nearest wall cell / 16, 7 the laminar channel profile
     6 s (1 - s), s = (y - 1/2) / (ny - 2), of mean 1 between the channel walls, times
-    (ny - 2) / (open cells of the column): a mass-conserving guess of u / u_in."""
+    (ny - 2) / (open cells of the column): a mass-conserving guess of u / u_in.
+    With prior (a (rho, vel) per case, e.g. coarse_prior()), three more channels
+    [n, nx, ny, 11]: 8, 9 the prior's u / u_in and 10 its (rho - 1) / u_in^2."""
     from scipy.ndimage import distance_transform_cdt
     nx, ny = cases[0].nx, cases[0].ny
-    out = np.empty((len(cases), nx, ny, 8), np.float32)
+    nf = 8 if prior is None else 11
+    out = np.empty((len(cases), nx, ny, nf), np.float32)
     xs = (np.arange(nx, dtype=np.float32) / nx)[:, None]
     ys = (np.arange(ny, dtype=np.float32) / ny)[None, :]
     sy = np.clip((np.arange(ny) - 0.5) / (ny - 2), 0.0, 1.0)
@@ -295,7 +379,7 @@ def features(cases) -> np.ndarray:
         # channel writes)
         c = cases[k]
         assert (c.nx, c.ny) == (nx, ny), "every case of one call has the same size"
-        planes = np.empty((8, nx, ny), np.float32)
+        planes = np.empty((nf, nx, ny), np.float32)
         planes[0] = c.mask
         planes[1] = c.u_in / 0.05
         planes[2] = (c.tau - 0.5) * 2.0
@@ -306,6 +390,9 @@ def features(cases) -> np.ndarray:
         open_cells = np.maximum((~c.mask).sum(1), 1).astype(np.float32)
         lam = prof[None, :] * ((ny - 2) / open_cells)[:, None]
         planes[7] = np.where(c.mask, 0.0, lam)
+        if prior is not None:
+            t = to_targets(c, *prior[k])
+            planes[8:11] = t.transpose(2, 0, 1)
         np.copyto(out[k], planes.transpose(1, 2, 0))
 
     _each(fill, range(len(cases)))
@@ -314,9 +401,12 @@ def features(cases) -> np.ndarray:
 
 def mirrored_features(x: np.ndarray) -> np.ndarray:
     """features() of the reflected cases from those of the cases: every channel reflects
-    with the geometry except y / ny (channel 5), which belongs to the lattice."""
+    with the geometry except y / ny (channel 5), which belongs to the lattice, and the
+    prior's u_y (channel 9), which also changes sign."""
     m = np.ascontiguousarray(x[:, :, ::-1])
     m[..., 5] = x[..., 5]
+    if x.shape[-1] > 9:
+        m[..., 9] *= -1
     return m
 
 
@@ -345,22 +435,29 @@ def generate(n: int, seed0: int = 0, batch: int = 64, max_steps: int = 60000,
 
 class Predictor:
     """The trained network and the per-channel normalisation of its targets: the network
-    learns (t - mean) / std of each target channel of to_targets()."""
+    learns (t - mean) / std of each target channel of to_targets(), or, with a coarse
+    prior (prior_factor > 0), of the difference between the targets and the prior's
+    (the network then corrects the upsampled coarse solution it is given as input)."""
 
-    def __init__(self, net, mean: np.ndarray, std: np.ndarray):
+    def __init__(self, net, mean: np.ndarray, std: np.ndarray, prior_factor: int = 0):
         self.net = net
         self.mean = np.asarray(mean, np.float32)
         self.std = np.asarray(std, np.float32)
+        self.prior_factor = prior_factor
 
     def predict(self, x: np.ndarray) -> np.ndarray:
-        """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8]."""
-        return self.net.predict(x) * self.std + self.mean
+        """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8 or 11]."""
+        t = self.net.predict(x) * self.std + self.mean
+        if self.prior_factor:
+            t = t + x[..., 8:11]
+        return t
 
     def predict_cases(self, cases, mirror: bool = False) -> np.ndarray:
-        """Targets for cases; with mirror, the mean of the prediction and the reflected
-        prediction of the reflected case (exact symmetry, so only the network's error
-        differs between the two)."""
-        x = features(cases)
+        """Targets for cases (with a prior model, the coarse solves run first); with
+        mirror, the mean of the prediction and the reflected prediction of the reflected
+        case (exact symmetry, so only the network's error differs between the two)."""
+        prior = coarse_prior(cases, self.prior_factor) if self.prior_factor else None
+        x = features(cases, prior)
         t = self.predict(x)
         if not mirror:
             return t
@@ -382,7 +479,8 @@ def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
 
 def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
           seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1, depth: int = 5,
-          mirror: bool = True, rho_weight: float = 1 / 9, time_budget: float | None = None):
+          mirror: bool = True, rho_weight: float = 1 / 9, time_budget: float | None = None,
+          prior_factor: int = 0):
     """Trains the network on solver-generated steady states with the library's kernels.
     Returns the Predictor and the per-epoch mean losses (on normalised targets).
 
@@ -391,23 +489,31 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
     fewer steps than velocity errors).  time_budget (seconds) caps the run: after the first
     epoch the epoch count, and with it the cosine schedule, shrinks to what fits.  With log,
     training runs log_every epochs at a time (same schedule as one call) and reports each
-    chunk's last epoch loss."""
+    chunk's last epoch loss.  prior_factor > 0: every case's steady state is also solved on
+    a lattice that many times coarser (coarse_prior), the upsampled result is an input and
+    the network learns the correction to it."""
     from . import Network
     c0 = cases[0]
-    net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed, depth=depth)
+    prior = coarse_prior(cases, prior_factor, dev=dev) if prior_factor else None
+    net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed, depth=depth,
+                  features=8 if prior is None else 11)
     if mirror:
         extra = [mirrored(c, *f) for c, f in zip(cases, fields)]
+        if prior is not None:
+            prior = list(prior) + [mirrored(c, *pf)[1] for c, pf in zip(cases, prior)]
         cases = list(cases) + [c for c, _ in extra]
         fields = list(fields) + [f for _, f in extra]
-    x = features(cases)
+    x = features(cases, prior)
     y = np.stack([to_targets(c, *f) for c, f in zip(cases, fields)])
+    if prior is not None:
+        y = y - x[..., 8:11]
     mean = y.reshape(-1, 3).mean(0)
     std = y.reshape(-1, 3).std(0) + 1e-6
     std[2] /= np.sqrt(rho_weight)
     y = ((y - mean) / std).astype(np.float32)
     net.set_data(x, y)
     del x, y
-    model = Predictor(net, mean, std)
+    model = Predictor(net, mean, std, prior_factor)
     if log is None and time_budget is None:
         return model, net.train(epochs, lr=lr, seed=seed)
     losses = []
@@ -428,13 +534,13 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
 @dataclass
 class Comparison:
     cold_steps: np.ndarray
-    warm_steps: np.ndarray       # steps of the warm run, or of warm + cold after a fallback
-    fallback: np.ndarray         # warm start rejected (its first residual was worse)
+    warm_steps: np.ndarray       # steps of the start that was kept (its cold twin's after a fallback)
+    fallback: np.ndarray         # warm start dropped (its residual was worse at the judging check)
     cold_seconds: float          # the cold batch's wall time
-    warm_seconds: float          # inference + the warm batch (+ the fallback's cold re-run)
+    warm_seconds: float          # inference + the warm batch (both twins until the judging check)
     inference_seconds: float
     cold_case_seconds: np.ndarray = None   # per case: steps to tol x time per case-step
-    warm_case_seconds: np.ndarray = None   # the same warm, plus its share of inference
+    warm_case_seconds: np.ndarray = None   # the same warm (both twins' steps) plus its share of inference
 
     def summary(self) -> dict:
         """Over the cases the cold run brings to the tolerance.  The batch wall times run
@@ -448,6 +554,9 @@ class Comparison:
                "warm_seconds": round(self.warm_seconds, 3),
                "batch_wall_ratio": round(self.warm_seconds / self.cold_seconds, 3),
                "inference_seconds": round(self.inference_seconds, 3)}
+        if getattr(self, "twin_steps", None) is not None:
+            charged = (self.warm_steps[ok] + self.twin_steps[ok]) / self.cold_steps[ok]
+            out["median_charged_step_ratio"] = float(np.median(charged))
         if self.cold_case_seconds is not None:
             w, c = self.warm_case_seconds[ok], self.cold_case_seconds[ok]
             out["time_to_residual_ratio"] = round(float(w.sum() / c.sum()), 4)
@@ -462,45 +571,45 @@ def compare(net, cases, tol: float = 1e-6, check_every: int = 500, max_steps: in
 
     The warm start is the network's (rho, u) (with mirror, averaged with the reflected
     prediction of the reflected case: about 1.5 % fewer steps for twice the inference, a net
-    loss in time) and, with neq, the non-equilibrium part of populations().  neq pays when the prediction is very close (a converged state restarts
-    in a third of the steps); at the network's present accuracy it changes the step count
-    by under 2 % and costs host time, so it is off by default.  Both runs are timed from
-    the set-up of their batch; the warm run also counts feature construction and
-    inference.  Both runs stop when every case is below tol.  After judge_at steps the
-    warm run's residual is compared with the cold run's at the same step count; a warm start
-    that is worse is rejected and that case is re-run cold, its cost counting the wasted
-    warm steps."""
+    loss in time) and, with neq, the non-equilibrium part of populations().  neq pays when
+    the prediction is very close (a converged state restarts in a third of the steps); at
+    the network's present accuracy it changes the step count by under 2 % and costs host
+    time, so it is off by default.
+
+    The warm run decides its own fallback, as a deployment would: each case runs from the
+    prediction and, beside it in the same batch, from the cold start; at the first check
+    at or after judge_at steps the start with the larger residual is dropped (the
+    north_star's "cold start if the initial residual is worse").  Every step of both twins
+    is charged to the warm start.  A separate, complete cold run gives the reference
+    times.  Both runs are timed from the set-up of their batch; the warm run also counts
+    feature construction and inference."""
+    n = len(cases)
     cold, _ = solve(cases, tol=tol, check_every=check_every, max_steps=max_steps, dev=dev,
                     keep_fields=False)
     t0 = time.perf_counter()
     pred = net.predict_cases(cases, mirror) if hasattr(net, "predict_cases") else \
         net.predict(features(cases))
-    init = _each(lambda k: from_targets(cases[k], pred[k]), range(len(cases)))
+    init = _each(lambda k: from_targets(cases[k], pred[k]), range(n))
     t_inf = time.perf_counter() - t0
-    warm, _ = solve(cases, init=init, tol=tol, check_every=check_every, max_steps=max_steps,
-                    dev=dev, keep_fields=False, neq=neq)
-    j = max(0, judge_at // check_every - 1)
-    rw = warm.history[min(j, len(warm.history) - 1)]
-    rc = cold.history[min(j, len(cold.history) - 1)]
-    # a case already converged by the judging step is never rejected
-    fallback = (rw > rc) & ~((warm.steps >= 0) & (warm.steps <= (j + 1) * check_every))
-    steps = warm.steps.copy()
-    secs = t_inf + warm.seconds
-    if fallback.any():
-        idx = np.flatnonzero(fallback)
-        redo, _ = solve([cases[i] for i in idx], tol=tol, check_every=check_every,
-                        max_steps=max_steps, dev=dev, keep_fields=False)
-        for k, i in enumerate(idx):
-            steps[i] = (j + 1) * check_every + redo.steps[k] if redo.steps[k] >= 0 else -1
-        secs += redo.seconds
+    warm, _ = solve(list(cases) + list(cases), init=list(init) + [None] * n, tol=tol,
+                    check_every=check_every, max_steps=max_steps, dev=dev, keep_fields=False,
+                    neq=neq, judge_twins_at=judge_at)
+    fallback = warm.rejected[:n]
+    steps = np.where(fallback, warm.steps[n:], warm.steps[:n])
+    # the dropped twin's steps: until the judging check, or its own convergence before it
+    lost = np.where(fallback, warm.steps[:n], warm.steps[n:])
+    judged = warm.judged_at if warm.judged_at else warm.steps_run
+    extra = np.where((lost > 0) & (lost < judged), lost, judged)
     # a warm run that misses the tolerance where the cold run reached it counts max_steps
     steps = np.where((steps < 0) & (cold.steps > 0), max_steps, steps)
-    # per-case time to the tolerance: its steps at the measured cost of one case-step (the
-    # same lattice in both runs), plus its share of the inference for the warm start
+    # per-case time to the tolerance: its steps (both twins') at the measured cost of one
+    # case-step, plus its share of the inference for the warm start
     per_cold = cold.seconds_per_case_step
     per_warm = warm.seconds_per_case_step
     cold_case = np.where(cold.steps > 0, cold.steps * per_cold, np.nan)
-    warm_case = np.where(steps > 0, steps * per_warm + t_inf / len(cases), np.nan)
-    cmp = Comparison(cold.steps, steps, fallback, cold.seconds, secs, t_inf, cold_case, warm_case)
-    cmp.warm_raw_steps = warm.steps
+    warm_case = np.where(steps > 0, (steps + extra) * per_warm + t_inf / n, np.nan)
+    cmp = Comparison(cold.steps, steps, fallback, cold.seconds, t_inf + warm.seconds, t_inf,
+                     cold_case, warm_case)
+    cmp.warm_raw_steps = warm.steps[:n]
+    cmp.twin_steps = extra
     return cmp

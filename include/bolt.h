@@ -225,6 +225,9 @@ int bolt_batch_reduce(bolt_batch* b, int inst, int what, double* out);
 int bolt_batch_cells(const bolt_batch* b, uint64_t* out);
 /* bolt_lattice_residual of one instance */
 int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double* out);
+/* bolt_lattice_residual of every active instance in one launch (out[n]; an instance's
+ * snapshot is shared with bolt_batch_residual); NaN for inactive instances. */
+int bolt_batch_residuals(bolt_batch* b, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
 /* instances that later steps advance (active[n], non-zero: advance); an inactive instance
@@ -239,7 +242,7 @@ int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
 /* ---- warm-start network (no counterpart in the reference) ----------------------------
  * A U-Net of `depth` 2x2 poolings predicting (rho, u) of a steady channel flow from per-cell
  * features, trained and evaluated with this library's bf16 MFMA kernels (float accumulation,
- * float master weights, Adam).  Host arrays are channels-last: inputs [n][nx][ny][8],
+ * float master weights, Adam).  Host arrays are channels-last: inputs [n][nx][ny][nfeat],
  * outputs [n][nx][ny][3]; nx and ny multiples of 2^depth.  Parameters are flattened per
  * layer (encoder pairs by level, bottleneck pair, decoder from the deepest level, 1x1
  * output) as W[cout][taps*cin] (tap = kx*3 + ky for 3x3 layers) followed by b[cout]. */
@@ -247,6 +250,11 @@ typedef struct bolt_nn bolt_nn;
 int bolt_nn_create(bolt_ctx* ctx, int nx, int ny, int batch, int width, int depth, uint64_t seed,
                    bolt_nn** out);
 int bolt_nn_destroy(bolt_nn* net);
+/* Feature channels per cell of the host inputs (1..16, default 8); set before
+ * bolt_nn_set_data, bolt_nn_loss_grad and bolt_nn_predict, whose x arrays are then
+ * [n][nx][ny][nfeat]. */
+int bolt_nn_set_features(bolt_nn* net, int nfeat);
+int bolt_nn_features(const bolt_nn* net, int* out);
 int bolt_nn_param_count(const bolt_nn* net, uint64_t* out);
 int bolt_nn_get_params(bolt_nn* net, float* out, uint64_t n);
 int bolt_nn_set_params(bolt_nn* net, const float* in, uint64_t n);

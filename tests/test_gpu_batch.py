@@ -179,3 +179,31 @@ def test_fluid_only_batch_matches_oracle_bit_exactly():
         np.testing.assert_array_equal(b.read(k, "f"), ref.f)
         nw = ~c.mask
         np.testing.assert_array_equal(b.read(k, "vel")[nw], ref.vel[nw])
+
+
+def test_batched_residuals_equal_per_instance_residuals():
+    # one launch for every active instance against bolt_batch_residual one by one, on two
+    # identical batches; maxima do not depend on the order of the reduction, so the values
+    # are equal, and an inactive instance reports NaN
+    sp = specs("trt")
+    a, b = build_batch(sp, "trt"), build_batch(sp, "trt")
+    for x in (a, b):
+        x.step(20)
+    ra = a.residuals(0)
+    rb = [b.residual(k, 0) for k in range(len(sp))]
+    assert np.isinf(ra).all() and np.isinf(rb).all()
+    for x in (a, b):
+        x.step(40)
+    ra = a.residuals(40)
+    rb = np.array([b.residual(k, 40) for k in range(len(sp))])
+    assert np.isfinite(ra).all() and (ra > 0).all()
+    np.testing.assert_array_equal(ra, rb)
+    active = np.array([True, False, True, True])
+    a.set_active(active)
+    b.set_active(active)
+    for x in (a, b):
+        x.step(20)
+    ra = a.residuals(20)
+    assert np.isnan(ra[1])
+    rb = np.array([b.residual(k, 20) for k in (0, 2, 3)])
+    np.testing.assert_array_equal(ra[[0, 2, 3]], rb)

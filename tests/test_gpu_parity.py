@@ -271,6 +271,36 @@ def test_simulation_serde():  # test_core.py:106-127
         assert sim2.iteration == 16
 
 
+@pytest.mark.parametrize("storage", ["f16s", "bf16s", "f64"])
+def test_checkpoint_round_trip_other_storage(storage):
+    # the checkpoint holds float populations, the reference's format (simulation.rs:130-176):
+    # double storage comes back rounded to float, and the 2-byte formats (f_i - w_i stored)
+    # pass through f_i = d + w_i in float and back, which restores the stored value up to
+    # one rounding of that float sum (about 3e-8 absolute)
+    bolt, core = _bolt()
+    with tempfile.TemporaryDirectory() as tmpdirname:
+        path = tmpdirname + "/checkpoint.mpk"
+        rng = np.random.default_rng(5)
+        sim1 = bolt.Simulation("gpu", [40, 30], core.trt(0.6), storage=storage)
+        sim1.fluid.vel[...] = (0.03 * rng.standard_normal((40, 30, 2))).astype(np.float32)
+        sim1.cells.flags[:, 0] = core.CellFlags.WALL
+        tr = sim1.add_tracer("T", core.bgk(0.7))
+        tr.val[...] = rng.random((40, 30)).astype(np.float32)
+        sim1.iterate(20)
+        sim1.write_checkpoint(path)
+        sim2 = bolt.Simulation.load_checkpoint("gpu", path, storage=storage)
+        assert sim2.storage == storage and sim2.iteration == 20
+        f1, f2 = sim1.read_field("f", dtype="f64"), sim2.read_field("f", dtype="f64")
+        assert np.abs(f1 - f2).max() <= 1e-7, np.abs(f1 - f2).max()
+        g1, g2 = sim1.read_field("g", "T", dtype="f64"), sim2.read_field("g", "T", dtype="f64")
+        assert np.abs(g1 - g2).max() <= 1e-7
+        # both continue alike
+        sim1.iterate(10)
+        sim2.iterate(10)
+        v1, v2 = sim1.read_field("vel", dtype="f64"), sim2.read_field("vel", dtype="f64")
+        assert np.abs(v1 - v2).max() <= 1e-6 * max(1.0, np.abs(v1).max() / 0.03)
+
+
 def test_odd_iterate_rejected_and_host_edits_after_start_ignored():
     bolt, core = _bolt()
     sim = bolt.Simulation("gpu", [16, 16], core.bgk(0.6))

@@ -39,10 +39,26 @@ def test_warm_start_needs_fewer_steps_than_cold_start():
     model, losses = ws.train(cases, fields, epochs=60, batch=8, width=32, depth=5, lr=2e-3, seed=0)
     assert np.isfinite(losses).all() and losses[-1] < 0.05 * losses[0], losses
     held_out = [ws.make_case(50_000 + s, **KW) for s in range(32)]
-    cmp = ws.compare(model, held_out, check_every=200, max_steps=40000, mirror=True)
+    cmp = ws.compare(model, held_out, check_every=200, max_steps=40000, mirror=True, judge_at=400)
     s = cmp.summary()
     print("warm start", s, flush=True)
     assert s["cases"] >= 24
     # measured about 0.78 (0.68 with ten times the training data); a network that predicted
-    # nothing useful scores about 1.0, and a rejected prediction costs judge_at extra steps
+    # nothing useful scores about 1.0, and a rejected prediction falls back to its cold twin
+    assert s["median_step_ratio"] < 0.85, s
+    # the fallback is decided in the warm run itself: a dropped start's twin carries on
+    assert (cmp.warm_steps[cmp.cold_steps > 0] > 0).all()
+
+
+def test_coarse_prior_warm_start():
+    # the network given the steady state of a 2x coarser lattice learns the correction to it
+    from boltzmann_amd import warmstart as ws
+    cases, fields, _ = ws.generate(400, seed0=0, batch=200, max_steps=40000, check_every=1000, **KW)
+    model, losses = ws.train(cases, fields, epochs=40, batch=8, width=32, depth=5, lr=2e-3, seed=0,
+                             prior_factor=2)
+    assert model.net.features == 11 and np.isfinite(losses).all()
+    held_out = [ws.make_case(50_000 + s, **KW) for s in range(32)]
+    s = ws.compare(model, held_out, check_every=200, max_steps=40000, judge_at=400).summary()
+    print("coarse-prior warm start", s, flush=True)
+    assert s["cases"] >= 24
     assert s["median_step_ratio"] < 0.85, s

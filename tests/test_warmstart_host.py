@@ -177,3 +177,48 @@ def test_features_of_a_batch_are_each_cases_own():
         open_cells = (~c.mask[blocked]).sum()
         np.testing.assert_allclose(lam[blocked][~c.mask[blocked]],
                                    prof[~c.mask[blocked]] * (c.ny - 2) / open_cells, rtol=1e-6)
+
+
+def test_coarse_case_keeps_reynolds_number_and_walls():
+    c = ws.make_case(5, nx=128, ny=64, size_range=(4, 10))
+    cc = ws.coarse_case(c, 2)
+    assert (cc.nx, cc.ny) == (64, 32) and cc.u_in == c.u_in and cc.re == c.re
+    # nu in coarse lattice units is half the fine one (same u_in, half the lengths)
+    assert (cc.tau - 0.5) == pytest.approx((c.tau - 0.5) / 2)
+    assert cc.mask[:, 0].all() and cc.mask[:, -1].all()
+    # a coarse cell is a wall exactly when at least half of its 2 x 2 fine cells are
+    frac = c.mask.reshape(64, 2, 32, 2).mean(axis=(1, 3))
+    inner = cc.mask[:, 1:-1]
+    np.testing.assert_array_equal(inner, (frac >= 0.5)[:, 1:-1])
+
+
+def test_upsample_reproduces_linear_fields_between_coarse_centres():
+    c = _case(48, 32)
+    f = 2
+    X, Y = np.meshgrid((np.arange(24) + 0.5) * f - 0.5, (np.arange(16) + 0.5) * f - 0.5, indexing="ij")
+    rho = (1 + 1e-3 * X - 2e-3 * Y).astype(np.float32)
+    vel = np.stack([0.01 + 1e-4 * X + 3e-4 * Y, -2e-4 * X + 1e-4 * Y], -1).astype(np.float32)
+    r, v = ws.upsample(c, f, rho, vel)
+    Xf, Yf = np.meshgrid(np.arange(48), np.arange(32), indexing="ij")
+    inside = (Xf >= 1) & (Xf <= 46) & (Yf >= 1) & (Yf <= 30)   # between coarse cell centres
+    np.testing.assert_allclose(r[inside], (1 + 1e-3 * Xf - 2e-3 * Yf)[inside], atol=1e-6)
+    np.testing.assert_allclose(v[..., 0][inside], (0.01 + 1e-4 * Xf + 3e-4 * Yf)[inside], atol=1e-7)
+    np.testing.assert_allclose(v[..., 1][inside], (-2e-4 * Xf + 1e-4 * Yf)[inside], atol=1e-7)
+    # non-finite coarse values give the cold start
+    vel[3, 3, 0] = np.nan
+    r2, v2 = ws.upsample(c, f, rho, vel)
+    cr, cv = c.cold_state()
+    np.testing.assert_array_equal(v2, cv)
+
+
+def test_mirrored_features_with_prior_equal_the_reflected_cases_features():
+    cases = [ws.make_case(17 + s, nx=128, ny=64, size_range=(4, 10)) for s in range(3)]
+    rng = np.random.default_rng(0)
+    prior = [(1 + 1e-3 * rng.standard_normal((128, 64))).astype(np.float32) for _ in cases]
+    prior = [(r, (0.01 * rng.standard_normal((128, 64, 2))).astype(np.float32)) for r in prior]
+    x = ws.features(cases, prior)
+    assert x.shape[-1] == 11
+    refl = [ws.mirrored(c, *p) for c, p in zip(cases, prior)]
+    xr = ws.features([c for c, _ in refl], [p for _, p in refl])
+    np.testing.assert_array_equal(ws.mirrored_features(x), xr)
+    np.testing.assert_array_equal(x[..., :8], ws.features(cases))
