@@ -284,16 +284,17 @@ def cylinder(steps_marks=(1000, 5000, 20000), tau_minus=0.52):
 
 def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
                       width: int = 32, depth: int = 6, max_steps: int = 60000,
-                      train_budget: float = 420.0):
+                      train_budget: float = 420.0, prior_factor: int = 2):
     """Learned warm start against the plain solver's time to steady state.
 
     D2Q9 TRT channels of 512 x 256 cells with 1-8 seeded obstacles (radius / half-size
     8-32), u_in in [0.02, 0.05], Re in [10, 100].  n_train cases (seeds 0..) are solved to
     the FP64 residual 1e-6 in batches; those that converge within max_steps train the
     network (with their mirror images across the channel) for up to `epochs` epochs within
-    train_budget seconds; n_test held-out cases
-    (seeds 10^6..) run cold and warm (prediction plus its Chapman-Enskog non-equilibrium
-    part) to the same residual."""
+    train_budget seconds, given the 2x coarser lattice's steady state as input with
+    prior_factor 2 (0: geometry features only); n_test held-out cases (seeds 10^6..) run
+    cold and warm to the same residual, the warm run deciding its fallback against a cold
+    twin in its own batch."""
     from boltzmann_amd import warmstart as ws
 
     def log(msg):
@@ -305,45 +306,55 @@ def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
     net, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=width, depth=depth,
-                           lr=2e-3, seed=0, log=log, time_budget=train_budget)
+                           lr=2e-3, seed=0, log=log, time_budget=train_budget,
+                           prior_factor=prior_factor)
     t_train = time.perf_counter() - t0
     del fields
     held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
-    cmp = ws.compare(net, held, check_every=500, max_steps=2 * max_steps)
+    cmp = ws.compare(net, held, check_every=500, max_steps=2 * max_steps, judge_at=500)
     s = cmp.summary()
     log(f"held-out: {s}")
     return {"train_cases": len(cases), "train_requested": n_train,
             "train_steps_median": int(np.median(steps)) if len(steps) else None,
             "generate_s": round(t_gen, 1), "train_s": round(t_train, 1),
-            "epochs": len(losses), "width": width, "depth": depth,
+            "epochs": len(losses), "width": width, "depth": depth, "prior_factor": prior_factor,
             "params": int(net.net.param_count),
             "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
             "held_out": s}
 
 
-def warmstart_small(n_train: int = 600, n_test: int = 32, epochs: int = 60):
-    """The learned warm start at the reduced size of its GPU test, bounded to a few minutes:
-    128 x 64 channels (obstacle size 4-10, Re 10-40), n_train solver-generated cases, a
-    depth-5 network without and with the coarse-lattice prior (the steady state on a 2x
-    coarser lattice, upsampled, given as input; the network learns the correction), each
-    compared with the cold start on n_test held-out cases to the 1e-6 residual.  Each warm
-    run decides its own fallback against a cold twin in the same batch (both twins'
-    steps charged); time_to_residual_ratio counts features, coarse solve and inference."""
+def warmstart_bounded(n_train: int = 1000, n_test: int = 48, epochs: int = 30,
+                      train_budget: float = 45.0):
+    """The learned warm start at the metric's size, bounded to about three minutes: 512 x 256
+    channels (1-8 obstacles of size 8-32, Re 10-100, u_in 0.02-0.05), n_train seeded cases
+    solved to the 1e-6 residual (those that converge train), a depth-6 network without and
+    with the coarse-lattice prior (the steady state on a 2x coarser lattice, upsampled, as
+    input; the network learns the correction), each compared with the cold start on n_test
+    held-out cases.  Each warm run decides its own fallback against a cold twin in the same
+    batch (both twins' steps charged); time_to_residual_ratio counts features, the coarse
+    solve and inference."""
     from boltzmann_amd import warmstart as ws
-    kw = dict(nx=128, ny=64, re_range=(10.0, 40.0), size_range=(4, 10))
+
+    def log(msg):
+        print(f"[bench] warm start: {msg}", file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    cases, fields, _ = ws.generate(n_train, seed0=0, batch=300, max_steps=40000, check_every=1000, **kw)
-    out = {"lattice": [128, 64], "train_cases": len(cases), "generate_s": round(time.perf_counter() - t0, 1)}
-    held = [ws.make_case(50_000 + s, **kw) for s in range(n_test)]
-    for name, pf in (("plain", 0), ("coarse_prior", 2)):
+    cases, fields, _ = ws.generate(n_train, seed0=0, batch=250, max_steps=60000, check_every=2000,
+                                   log=log)
+    out = {"lattice": [512, 256], "train_cases": len(cases), "train_requested": n_train,
+           "generate_s": round(time.perf_counter() - t0, 1)}
+    held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
+    for name, pf in (("coarse_prior", 2), ("plain", 0)):
         t0 = time.perf_counter()
-        model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=32, depth=5, lr=2e-3,
-                                 seed=0, prior_factor=pf)
+        model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=32, depth=6, lr=2e-3,
+                                 seed=0, prior_factor=pf, time_budget=train_budget, log=log,
+                                 log_every=10)
         t_train = time.perf_counter() - t0
-        s = ws.compare(model, held, check_every=200, max_steps=40000, judge_at=400).summary()
+        s = ws.compare(model, held, check_every=500, max_steps=120000, judge_at=500).summary()
         s["train_s"] = round(t_train, 1)
+        s["epochs"] = len(losses)
         out[name] = s
-        print(f"[bench] warm start 128x64 {name}: {s}", file=sys.stderr, flush=True)
+        log(f"{name}: {s}")
         del model
     return out
 
@@ -471,8 +482,8 @@ def main() -> int:
     ap.add_argument("--ws-epochs", type=int, default=20)
     ap.add_argument("--nn-throughput", action="store_true",
                     help="add the warm-start network's training / inference throughput")
-    ap.add_argument("--no-warmstart-small", action="store_true",
-                    help="skip the bounded 128x64 warm-start leg of the default run")
+    ap.add_argument("--no-warmstart-bounded", action="store_true",
+                    help="skip the bounded 512x256 warm-start leg of the default run (~3 min)")
     ap.add_argument("--vw-sweep", action="store_true",
                     help="add MLUPS of every storage format at 1, 2 and 4 cells per lane")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -529,8 +540,8 @@ def main() -> int:
         print(f"[bench] rb batch: {variants['rb_batch']['mlups']:.0f} MLUPS", file=sys.stderr,
               flush=True)
         variants["f16s_over_f32"] = round(variants["trt_f16s"]["mlups"] / head["mlups"], 3)
-        if not args.no_warmstart_small:
-            variants["warmstart_channel_128x64"] = warmstart_small()
+        if not args.no_warmstart_bounded:
+            variants["warmstart_channel_512x256"] = warmstart_bounded()
     if args.vw_sweep:
         sweep = {}
         for st in ("f32", "f16s", "bf16s", "f32f", "f64"):
@@ -544,7 +555,7 @@ def main() -> int:
         variants["nn_unet_512x256"] = nn_throughput()
         print(f"[bench] network: {variants['nn_unet_512x256']}", file=sys.stderr, flush=True)
     if args.warmstart:
-        variants["warmstart_channel_512x256"] = warmstart_channel(
+        variants["warmstart_channel_512x256_full"] = warmstart_channel(
             args.ws_train, args.ws_test, args.ws_epochs)
 
     achieved = head["mlups"] * 1e6 * B_ALG["f32"] / 1e9  # GB/s
