@@ -453,14 +453,22 @@ static int pick_block(const bolt_lattice* L) {
     return 64;
 }
 
-// Cells per lane of the fluid step.  One cell per lane unless set_vector_width asked for
-// 2 or 4: on MI355X the one-cell kernel measured fastest for every storage format (D3Q27
-// TRT 256^3: f32 24.1k MLUPS against 20.6k / 22.2k at 2 / 4 cells per lane, f16s 43.1k
-// against 31.4k / 40.5k, f64 11.1k against 11.0k).  A fused tracer always runs one cell per
-// lane.
-static int pick_vw(const bolt_lattice* L, bool fused_tracer) {
-    if (fused_tracer || L->vw <= 1) return 1;
-    return L->n[L->ndim - 1] % L->vw == 0 ? L->vw : 1;
+// Cells per lane of the fluid step on even or odd steps.  set_vector_width forces one choice
+// for both.  The built-in choice, from the per-launch kernel trace on MI355X (D3Q27 TRT
+// 256^3): the odd (own-cell) step of the 2-byte formats runs 2 cells per lane, one 4-byte
+// access per plane (f16s 353.5 us against 375.0 us at one cell per lane, bf16s 356.5
+// against 378.6); everything else runs one cell per lane, whose even (streaming) step and
+// float/double odd step were the fastest (f32 730.1 / 675.6 us against 938.9 / 702.9 at 2
+// cells and 802.8 / 706.8 at 4).  A fused tracer always runs one cell per lane.
+static int pick_vw(const bolt_lattice* L, bool fused_tracer, bool even) {
+    if (fused_tracer) return 1;
+    const int len = L->n[L->ndim - 1];
+    if (L->vw == 0) {
+        const bool half = L->storage == kF16S || L->storage == kBF16S;
+        return (half && !even && len % 2 == 0 && len >= 64) ? 2 : 1;
+    }
+    if (L->vw <= 1) return 1;
+    return len % L->vw == 0 ? L->vw : 1;
 }
 
 static void fill_geometry(const bolt_lattice* L, StepParams& p, int block) {
@@ -930,21 +938,21 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     StepFn fo = lookup_step(L->ndim, L->q, L->collision, L->storage, false, tq);
     if (!fe || !fo) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d collision %d storage %d",
                                 L->ndim, L->q, L->collision, L->storage);
-    const int vw = pick_vw(L, can_fuse);
-    const WideGeom wg = make_wide_geom(L->ncell, p.len, vw);
+    const int vwe = pick_vw(L, can_fuse, true), vwo = pick_vw(L, can_fuse, false);
+    const WideGeom wge = make_wide_geom(L->ncell, p.len, vwe), wgo = make_wide_geom(L->ncell, p.len, vwo);
     WideFn we = nullptr, wo = nullptr;
-    if (vw > 1) {
-        we = lookup_wide(L->ndim, L->q, L->collision, L->storage, vw, true);
-        wo = lookup_wide(L->ndim, L->q, L->collision, L->storage, vw, false);
-        if (!we || !wo) return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vw, L->storage);
-    }
+    if (vwe > 1 && !(we = lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true)))
+        return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwe, L->storage);
+    if (vwo > 1 && !(wo = lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
+        return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwo, L->storage);
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
-    const dim3 wgrid((unsigned)wg.nblocks), wblk(256);
+    const dim3 wgride((unsigned)wge.nblocks), wgrido((unsigned)wgo.nblocks), wblk(256);
     hipStream_t s = L->ctx->stream;
     for (uint64_t it = 0; it < n; ++it) {
         const bool even = (it % 2) == 0;
         p.write_macro = (it + 1 == n) || sep_tracers;
-        if (vw > 1) hipLaunchKernelGGL(even ? we : wo, wgrid, wblk, 0, s, p, wg);
+        if (even && vwe > 1) hipLaunchKernelGGL(we, wgride, wblk, 0, s, p, wge);
+        else if (!even && vwo > 1) hipLaunchKernelGGL(wo, wgrido, wblk, 0, s, p, wgo);
         else hipLaunchKernelGGL(even ? fe : fo, grid, blk, 0, s, p);
         for (size_t k = first_sep; k < L->tracers.size(); ++k) {
             const Tracer& t = L->tracers[k];

@@ -193,10 +193,11 @@ int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);
  * 0 restores the built-in choice. */
 int bolt_lattice_set_block(bolt_lattice* lat, int block);
 /* Launch tuning: cells of the contiguous axis per lane in the fluid step (1, 2 or 4; 2 at
- * most for double storage; the axis must be a multiple of it); 0 restores the built-in
- * choice, one cell per lane (measured fastest on MI355X).  Wider lanes make every
- * population access an aligned 8- or 16-byte vector and do the streaming shift along the
- * row with lane rotations.  A tracer fused into the fluid step always runs one cell per
+ * most for double storage; the axis must be a multiple of it), on every step; 0 restores
+ * the built-in choice, measured fastest on MI355X: 2 cells per lane on the odd (own-cell)
+ * steps of the 2-byte formats, one cell per lane otherwise.  Wider lanes make every
+ * population access an aligned 4-, 8- or 16-byte vector and do the streaming shift along
+ * the row with lane rotations.  A tracer fused into the fluid step always runs one cell per
  * lane.  Results do not depend on the choice (bit-identical). */
 int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
 

@@ -125,3 +125,15 @@ def test_vector_width_is_validated():
         d.set_vector_width(4)
     s.set_vector_width(2)
     s.set_vector_width(0)
+
+
+@pytest.mark.parametrize("counts", [(7, 256), (5, 200), (3, 4, 256), (2, 3, 68)])
+@pytest.mark.parametrize("storage", ["f16s", "bf16s"])
+def test_default_widths_equal_one_cell_per_lane(counts, storage):
+    # the built-in choice mixes kernels (odd steps of the 2-byte formats at 2 cells per lane)
+    (one, auto), flags = _pair_sims(counts, OMEGAS[1], storage, seed=21, gravity=True, vw=2)
+    auto.set_vector_width(0)
+    for n in (2, 4, 6):
+        one.iterate(n)
+        auto.iterate(n)
+        _same(one, auto)
