@@ -284,15 +284,15 @@ def cylinder(steps_marks=(1000, 5000, 20000), tau_minus=0.52):
 
 def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
                       width: int = 32, depth: int = 6, max_steps: int = 60000,
-                      train_budget: float = 420.0, prior_factor: int = 2):
+                      train_budget: float = 420.0, prior_factor: int = 4):
     """Learned warm start against the plain solver's time to steady state.
 
     D2Q9 TRT channels of 512 x 256 cells with 1-8 seeded obstacles (radius / half-size
     8-32), u_in in [0.02, 0.05], Re in [10, 100].  n_train cases (seeds 0..) are solved to
     the FP64 residual 1e-6 in batches; those that converge within max_steps train the
     network (with their mirror images across the channel) for up to `epochs` epochs within
-    train_budget seconds, given the 2x coarser lattice's steady state as input with
-    prior_factor 2 (0: geometry features only); n_test held-out cases (seeds 10^6..) run
+    train_budget seconds, given the 4x coarser lattice's steady state as input with
+    prior_factor 4 (0: geometry features only); n_test held-out cases (seeds 10^6..) run
     cold and warm to the same residual, the warm run deciding its fallback against a cold
     twin in its own batch."""
     from boltzmann_amd import warmstart as ws
@@ -328,7 +328,7 @@ def warmstart_bounded(n_train: int = 1000, n_test: int = 48, epochs: int = 30,
     """The learned warm start at the metric's size, bounded to about three minutes: 512 x 256
     channels (1-8 obstacles of size 8-32, Re 10-100, u_in 0.02-0.05), n_train seeded cases
     solved to the 1e-6 residual (those that converge train), a depth-6 network without and
-    with the coarse-lattice prior (the steady state on a 2x coarser lattice, upsampled, as
+    with the coarse-lattice prior (the steady state on a 4x coarser lattice, upsampled, as
     input; the network learns the correction), each compared with the cold start on n_test
     held-out cases.  Each warm run decides its own fallback against a cold twin in the same
     batch (both twins' steps charged); time_to_residual_ratio counts features, the coarse
@@ -344,7 +344,7 @@ def warmstart_bounded(n_train: int = 1000, n_test: int = 48, epochs: int = 30,
     out = {"lattice": [512, 256], "train_cases": len(cases), "train_requested": n_train,
            "generate_s": round(time.perf_counter() - t0, 1)}
     held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
-    for name, pf in (("coarse_prior", 2), ("plain", 0)):
+    for name, pf in (("coarse_prior", 4), ("plain", 0)):
         t0 = time.perf_counter()
         model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=32, depth=6, lr=2e-3,
                                  seed=0, prior_factor=pf, time_budget=train_budget, log=log,

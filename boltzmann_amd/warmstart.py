@@ -281,14 +281,21 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
     rejected = np.zeros(n, bool)
     judged_at = 0
     npair = n // 2 if judge_twins_at > 0 else 0
+    timings = {"set_up": time.perf_counter() - t0, "step": 0.0, "residual": 0.0, "set_active": 0.0}
     while done < max_steps and active.any():
         case_steps += int(active.sum()) * check_every
+        t1 = time.perf_counter()
         b.step(check_every - 100)
+        t2 = time.perf_counter()
         b.residuals(0)                # snapshot u^(n-100) of every active case
+        t3 = time.perf_counter()
         b.step(100)
         done += check_every
         checks += 1
+        t4 = time.perf_counter()
         rs = b.residuals(100)
+        timings["step"] += (t2 - t1) + (t4 - t3)
+        timings["residual"] += (t3 - t2) + (time.perf_counter() - t4)
         for k in np.flatnonzero(active):
             r = rs[k]
             last[k] = r
@@ -318,11 +325,15 @@ def solve(cases, init=None, tol=1e-6, check_every=1000, max_steps=100000, dev="g
             leave |= (steps < 0) & (checks - best_at >= drop_stalled)
         if (active & leave).any():
             active &= ~leave
+            t1 = time.perf_counter()
             b.set_active(active)
+            timings["set_active"] += time.perf_counter() - t1
     secs = time.perf_counter() - t0
     res = SolveResult(steps, last, secs, first, np.array(hist), check_every, done, case_steps)
     res.rejected = rejected
     res.judged_at = judged_at
+    res.timings = timings  # wall seconds: set-up, stepping (b.step: the launches, synchronous
+    #                       only when a later call waits), residual checks, active-set changes
     if not keep_fields:
         return res, None
     fields = [(b.read(k, "rho"), b.read(k, "vel")) for k in range(n)]
@@ -439,11 +450,13 @@ class Predictor:
     prior (prior_factor > 0), of the difference between the targets and the prior's
     (the network then corrects the upsampled coarse solution it is given as input)."""
 
-    def __init__(self, net, mean: np.ndarray, std: np.ndarray, prior_factor: int = 0):
+    def __init__(self, net, mean: np.ndarray, std: np.ndarray, prior_factor: int = 0,
+                 prior_tol: float = 1e-5):
         self.net = net
         self.mean = np.asarray(mean, np.float32)
         self.std = np.asarray(std, np.float32)
         self.prior_factor = prior_factor
+        self.prior_tol = prior_tol
 
     def predict(self, x: np.ndarray) -> np.ndarray:
         """Targets [n, nx, ny, 3] (as to_targets) for inputs [n, nx, ny, 8 or 11]."""
@@ -456,7 +469,7 @@ class Predictor:
         """Targets for cases (with a prior model, the coarse solves run first); with
         mirror, the mean of the prediction and the reflected prediction of the reflected
         case (exact symmetry, so only the network's error differs between the two)."""
-        prior = coarse_prior(cases, self.prior_factor) if self.prior_factor else None
+        prior = coarse_prior(cases, self.prior_factor, self.prior_tol) if self.prior_factor else None
         x = features(cases, prior)
         t = self.predict(x)
         if not mirror:
@@ -480,7 +493,7 @@ def mirrored(case: Case, rho: np.ndarray, vel: np.ndarray):
 def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: float = 1e-3,
           seed: int = 0, dev: str = "gpu", log=None, log_every: int = 1, depth: int = 5,
           mirror: bool = True, rho_weight: float = 1 / 9, time_budget: float | None = None,
-          prior_factor: int = 0):
+          prior_factor: int = 0, prior_tol: float = 1e-5):
     """Trains the network on solver-generated steady states with the library's kernels.
     Returns the Predictor and the per-epoch mean losses (on normalised targets).
 
@@ -490,11 +503,11 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
     epoch the epoch count, and with it the cosine schedule, shrinks to what fits.  With log,
     training runs log_every epochs at a time (same schedule as one call) and reports each
     chunk's last epoch loss.  prior_factor > 0: every case's steady state is also solved on
-    a lattice that many times coarser (coarse_prior), the upsampled result is an input and
-    the network learns the correction to it."""
+    a lattice that many times coarser (coarse_prior, to prior_tol), the upsampled result is
+    an input and the network learns the correction to it."""
     from . import Network
     c0 = cases[0]
-    prior = coarse_prior(cases, prior_factor, dev=dev) if prior_factor else None
+    prior = coarse_prior(cases, prior_factor, prior_tol, dev=dev) if prior_factor else None
     net = Network(dev, c0.nx, c0.ny, batch, width=width, seed=seed, depth=depth,
                   features=8 if prior is None else 11)
     if mirror:
@@ -513,7 +526,7 @@ def train(cases, fields, epochs: int = 30, batch: int = 8, width: int = 32, lr: 
     y = ((y - mean) / std).astype(np.float32)
     net.set_data(x, y)
     del x, y
-    model = Predictor(net, mean, std, prior_factor)
+    model = Predictor(net, mean, std, prior_factor, prior_tol)
     if log is None and time_budget is None:
         return model, net.train(epochs, lr=lr, seed=seed)
     losses = []
