@@ -27,13 +27,16 @@
 //   orc_update_ns_f64 / orc_update_ad_f64: the same update carried out in double
 //                   throughout (double storage); the FP64 comparison run.
 //
-// How each part is pinned (tests/test_oracle_*.py): the velocity tables against the
+// How each part is pinned (tests/test_oracle.py): the velocity tables against the
 // fixture parsed from velocities.rs; the omega=0 two-step map against a particle model
 // of free streaming with full-way bounce-back; the full update against a textbook
 // push/pull stream-and-collide in numpy float64 (np.roll streaming, generic
 // equilibrium, bounce-back walls, equilibrium-Dirichlet fixed cells, velocity-shift
-// forcing); equilibrium moments against their closed forms; TRT(w,w) == BGK(w); TRT
-// linearity in (w+, w-); the reference's own integration asserts (test_core.py:57-127).
+// forcing) -- to 2e-6 for the float kernels and to 1e-13 for both double paths
+// (orc_update_ns_f64, and orc_update_ad_f64 on its own with a moving fluid); equilibrium
+// moments against their closed forms; TRT(w,w) == BGK(w); TRT linearity in (w+, w-); the
+// reference's own integration asserts (test_core.py:57-127); orc_equilibrate against the
+// product's host equilibrium (tests/test_host_sequence.py).
 
 #include <cmath>
 #include <cstdint>
