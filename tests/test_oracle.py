@@ -354,3 +354,28 @@ def test_gaussian_tracer_diffusivity(d):
 def test_taylor_green_energy_decay(coll):
     got, want = cf.taylor_green(cf.OracleBackend, coll, n=32, dt=200)
     assert abs(got / want - 1) < 1e-2, (got, want)
+
+
+def test_oracle_equilibrium_is_the_float32_formula_in_order():
+    # orc_equilibrate is velocities.rs:43-52 evaluated in float32 left to right:
+    # rho * w * (1 + 3 vq + 4.5 vq vq - 1.5 vv), vq and vv accumulated from 0 in component
+    # order; numpy float32 scalars replay that order operation by operation
+    rng = np.random.default_rng(0)
+    rho = (1 + 0.1 * rng.standard_normal(50)).astype(np.float32)
+    u = (0.1 * rng.standard_normal((50, 3))).astype(np.float32)
+    f = oracle.equilibrate(27, rho, u)
+    c, w, _ = oracle.velocity_set(3, 27)
+    one, three, k45, k15 = (np.float32(x) for x in (1.0, 3.0, 4.5, 1.5))
+    for n in range(50):
+        vv = np.float32(0)
+        for a in range(3):
+            vv = np.float32(vv + u[n, a] * u[n, a])
+        for i in range(27):
+            vq = np.float32(0)
+            for a in range(3):
+                vq = np.float32(vq + u[n, a] * np.float32(c[i, a]))
+            val = np.float32(np.float32(rho[n] * w[i]) *
+                             np.float32(np.float32(np.float32(one + np.float32(three * vq)) +
+                                                   np.float32(np.float32(k45 * vq) * vq)) -
+                                        np.float32(k15 * vv)))
+            assert f[n, i] == val
