@@ -283,7 +283,7 @@ def cylinder(steps_marks=(1000, 5000, 20000), tau_minus=0.52):
 
 
 def warmstart_channel(n_train: int = 2000, n_test: int = 100, epochs: int = 20,
-                      width: int = 32, depth: int = 6, max_steps: int = 60000,
+                      width: int = 16, depth: int = 6, max_steps: int = 60000,
                       train_budget: float = 420.0, prior_factor: int = 4):
     """Learned warm start against the plain solver's time to steady state.
 
@@ -327,7 +327,8 @@ def warmstart_bounded(n_train: int = 1000, n_test: int = 48, epochs: int = 30,
                       train_budget: float = 45.0):
     """The learned warm start at the metric's size, bounded to about three minutes: 512 x 256
     channels (1-8 obstacles of size 8-32, Re 10-100, u_in 0.02-0.05), n_train seeded cases
-    solved to the 1e-6 residual (those that converge train), a depth-6 network without and
+    solved to the 1e-6 residual (those that converge train), a depth-6 width-16 network
+    (0.64 M parameters) without and
     with the coarse-lattice prior (the steady state on a 4x coarser lattice, upsampled, as
     input; the network learns the correction), each compared with the cold start on n_test
     held-out cases.  Each warm run decides its own fallback against a cold twin in the same
@@ -346,13 +347,14 @@ def warmstart_bounded(n_train: int = 1000, n_test: int = 48, epochs: int = 30,
     held = [ws.make_case(10 ** 6 + s) for s in range(n_test)]
     for name, pf in (("coarse_prior", 4), ("plain", 0)):
         t0 = time.perf_counter()
-        model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=32, depth=6, lr=2e-3,
+        model, losses = ws.train(cases, fields, epochs=epochs, batch=8, width=16, depth=6, lr=2e-3,
                                  seed=0, prior_factor=pf, time_budget=train_budget, log=log,
                                  log_every=10)
         t_train = time.perf_counter() - t0
         s = ws.compare(model, held, check_every=500, max_steps=120000, judge_at=500).summary()
         s["train_s"] = round(t_train, 1)
         s["epochs"] = len(losses)
+        s["params"] = int(model.net.param_count)
         out[name] = s
         log(f"{name}: {s}")
         del model
