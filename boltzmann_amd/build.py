@@ -2,7 +2,8 @@
 
     python boltzmann_amd/build.py
 
-* libbolt_hip.so   -- HIP kernels for gfx950 and the C ABI of include/bolt.h (hipcc)
+* libbolt_hip.so   -- HIP kernels for gfx950 and the C ABI of include/bolt.h (hipcc; the
+                     translation units compile in parallel into _build/ and are linked)
 * _boltzmann*.so   -- the Python module over that ABI (g++ + pybind11)
 * bw_probe         -- memory-ceiling probe for the update's access pattern (bench.py --probe)
 
@@ -28,15 +29,37 @@ HIP_FLAGS = [
 ]
 
 
-def _run(cmd: list[str]) -> None:
-    print("+", " ".join(cmd), flush=True)
-    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    out = "\n".join(line for line in res.stdout.splitlines()
+def _report(cmd: list[str], rc: int, output: str) -> None:
+    out = "\n".join(line for line in output.splitlines()
                     if "packed-fp32-ops" not in line and line.strip())
     if out:
         print(out, flush=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"build step failed ({res.returncode}): {cmd[0]}")
+    if rc != 0:
+        raise RuntimeError(f"build step failed ({rc}): {' '.join(cmd)}")
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    _report(cmd, res.returncode, res.stdout)
+
+
+def _run_parallel(cmds: list[list[str]]) -> None:
+    """Runs the commands at the same time (one per translation unit); fails if any fails."""
+    procs = []
+    for cmd in cmds:
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+    errors = []
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        try:
+            _report(cmd, p.returncode, out)
+        except RuntimeError as e:
+            errors.append(str(e))
+    if errors:
+        raise RuntimeError("; ".join(errors))
 
 
 def lib_path() -> str:
@@ -48,18 +71,31 @@ def ext_path() -> str:
     return os.path.join(HERE, "_boltzmann" + suffix)
 
 
-def build_hip() -> None:
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", os.path.join(CSRC, "bolt_runtime.hip"),
-          os.path.join(CSRC, "bolt_nn.hip"), "-o", lib_path()])
+HIP_UNITS = ("bolt_runtime.hip", "bolt_wide.hip", "bolt_nn.hip")
+
+
+def build_hip(extra: list[list[str]] = ()) -> None:
+    """Compiles the translation units of libbolt_hip.so in parallel (with any `extra`
+    independent commands), then links them."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(u)[0] + ".o") for u in HIP_UNITS]
+    _run_parallel([[HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", os.path.join(CSRC, u), "-o", o]
+                   for u, o in zip(HIP_UNITS, objs)] + list(extra))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", *objs, "-o", lib_path()])
 
 
 def probe_path() -> str:
     return os.path.join(HERE, "bw_probe")
 
 
+def probe_cmd() -> list[str]:
+    return [HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], os.path.join(CSRC, "bw_probe.hip"),
+            "-o", probe_path()]
+
+
 def build_probe() -> None:
-    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], os.path.join(CSRC, "bw_probe.hip"),
-          "-o", probe_path()])
+    _run(probe_cmd())
 
 
 def build_ext() -> None:
@@ -72,9 +108,8 @@ def build_ext() -> None:
 
 
 def build() -> None:
-    build_hip()
+    build_hip(extra=[probe_cmd()])
     build_ext()
-    build_probe()
 
 
 if __name__ == "__main__":

@@ -24,7 +24,7 @@
 
 #include "../../include/bolt.h"
 #include "update.hpp"
-#include "update_wide.hpp"
+#include "update_wide.hpp"  // WideGeom and the launch geometry; kernels in bolt_wide.hip
 
 using namespace bolt;
 
@@ -140,33 +140,9 @@ static StepFn lookup_step(int d, int q, int coll, int s, bool even, int tq) {
     return nullptr;
 }
 
-// wide fluid steps (update_wide.hpp): VW cells per lane, no fused tracer
+// wide fluid steps (update_wide.hpp, instantiated in bolt_wide.hip)
 using WideFn = void (*)(StepParams, WideGeom);
-
-template <int D, int Q, int C, int S, int VW>
-static WideFn wide_fn(bool even) {
-    return even ? (WideFn)lbm_step_wide<D, Q, C, S, true, VW> : (WideFn)lbm_step_wide<D, Q, C, S, false, VW>;
-}
-
-template <int D, int Q, int C>
-static WideFn wide_fn_s(int s, int vw, bool even) {
-    switch (s) {
-        case kF32: return vw == 4 ? wide_fn<D, Q, C, kF32, 4>(even) : wide_fn<D, Q, C, kF32, 2>(even);
-        case kF32F: return vw == 4 ? wide_fn<D, Q, C, kF32F, 4>(even) : wide_fn<D, Q, C, kF32F, 2>(even);
-        case kF16S: return vw == 4 ? wide_fn<D, Q, C, kF16S, 4>(even) : wide_fn<D, Q, C, kF16S, 2>(even);
-        case kBF16S: return vw == 4 ? wide_fn<D, Q, C, kBF16S, 4>(even) : wide_fn<D, Q, C, kBF16S, 2>(even);
-        case kF64: return vw == 2 ? wide_fn<D, Q, C, kF64, 2>(even) : nullptr;
-    }
-    return nullptr;
-}
-
-static WideFn lookup_wide(int d, int q, int coll, int s, int vw, bool even) {
-    if (vw != 2 && vw != 4) return nullptr;
-    if (d == 2 && q == 9) return coll == 0 ? wide_fn_s<2, 9, 0>(s, vw, even) : wide_fn_s<2, 9, 1>(s, vw, even);
-    if (d == 3 && q == 27)
-        return coll == 0 ? wide_fn_s<3, 27, 0>(s, vw, even) : wide_fn_s<3, 27, 1>(s, vw, even);
-    return nullptr;
-}
+WideFn bolt_lookup_wide(int d, int q, int coll, int s, int vw, bool even);
 
 static StepFn lookup_ad(int d, int tq, int s, bool even) {
     if (d == 2 && tq == 5) {
@@ -941,9 +917,9 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const int vwe = pick_vw(L, can_fuse, true), vwo = pick_vw(L, can_fuse, false);
     const WideGeom wge = make_wide_geom(L->ncell, p.len, vwe), wgo = make_wide_geom(L->ncell, p.len, vwo);
     WideFn we = nullptr, wo = nullptr;
-    if (vwe > 1 && !(we = lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true)))
+    if (vwe > 1 && !(we = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwe, L->storage);
-    if (vwo > 1 && !(wo = lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
+    if (vwo > 1 && !(wo = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwo, L->storage);
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
     const dim3 wgride((unsigned)wge.nblocks), wgrido((unsigned)wgo.nblocks), wblk(256);
