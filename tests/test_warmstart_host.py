@@ -222,3 +222,91 @@ def test_mirrored_features_with_prior_equal_the_reflected_cases_features():
     xr = ws.features([c for c, _ in refl], [p for _, p in refl])
     np.testing.assert_array_equal(ws.mirrored_features(x), xr)
     np.testing.assert_array_equal(x[..., :8], ws.features(cases))
+
+
+class _ScriptedBatch:
+    """A stand-in for boltzmann_amd.Batch: residuals(k) follow a per-instance geometric decay
+    r0 * q^checks, stepping does nothing, and calls are recorded."""
+
+    def __init__(self, r0, q):
+        self.r0, self.q = np.asarray(r0, float), np.asarray(q, float)
+        self.checks = np.zeros(len(self.r0))
+        self.active = np.ones(len(self.r0), bool)
+        self.active_calls = []
+
+    def write(self, k, field, arr):
+        pass
+
+    def equilibrate(self):
+        pass
+
+    def step(self, n):
+        pass
+
+    def residuals(self, steps_between):
+        if steps_between == 0:
+            return np.full(len(self.r0), np.inf)
+        self.checks += self.active
+        return np.where(self.active, self.r0 * self.q ** self.checks, np.nan)
+
+    def set_active(self, active):
+        self.active = np.asarray(active, bool).copy()
+        self.active_calls.append(self.active.copy())
+
+    def read(self, k, field):
+        return None
+
+
+def test_twins_judged_at_the_first_check_after_judge_at():
+    # instances 0..2 start from predictions, 3..5 are their cold twins; residuals decay by 0.5
+    # per check, from different starts: twin pair 0 -- prediction better, pair 1 -- cold
+    # better, pair 2 -- prediction already below tol at the first check (it stays)
+    cases = [ws.make_case(60 + s, nx=64, ny=32, size_range=(4, 6)) for s in range(3)]
+    b = _ScriptedBatch(r0=[1e-4, 1e-2, 2e-7, 1e-3, 1e-4, 1e-3], q=[0.5] * 6)
+    init = [c.cold_state() for c in cases] + [None] * 3
+    res, _ = ws.solve(cases + cases, init=init, tol=1e-6, check_every=200, max_steps=10000,
+                      batch=b, keep_fields=False, judge_twins_at=500)
+    assert res.judged_at == 600                     # first check at or after 500 steps
+    np.testing.assert_array_equal(res.rejected, [False, True, False, True, False, True])
+    # every case reaches tol from the start that was kept; dropped twins stop early
+    assert (res.steps[[0, 2, 4]] > 0).all()
+    assert res.steps[2] == 200
+    first_drop = b.active_calls[0]
+    assert not first_drop[2]                          # converged at the first check
+    # case-steps count both twins until the judging check
+    assert res.case_steps >= 6 * 600 - 200
+
+
+def test_compare_charges_both_twins(monkeypatch):
+    # compare() with a scripted solver: the warm run keeps the better start of each pair and
+    # its per-case time counts the dropped twin's steps up to the judging check
+    cases = [ws.make_case(70 + s, nx=64, ny=32, size_range=(4, 6)) for s in range(2)]
+
+    class Net:
+        def predict(self, x):
+            return np.zeros(x.shape[:3] + (3,), np.float32)
+
+    calls = []
+
+    def fake_solve(cs, init=None, judge_twins_at=0, **kw):
+        calls.append(judge_twins_at)
+        n = len(cs)
+        if judge_twins_at:
+            r = ws.SolveResult(np.array([1000, 4000, 3000, 2500]), np.zeros(n), 2.0, np.zeros(n),
+                               check_every=500, steps_run=4000, case_steps=10000)
+            r.rejected = np.array([False, True, True, False])
+            r.judged_at = 500
+        else:
+            r = ws.SolveResult(np.array([3000, 2500]), np.zeros(n), 1.0, np.zeros(n),
+                               check_every=500, steps_run=3000, case_steps=5500)
+        return r, None
+
+    monkeypatch.setattr(ws, "solve", fake_solve)
+    cmp = ws.compare(Net(), cases, judge_at=500)
+    assert calls == [0, 500]
+    np.testing.assert_array_equal(cmp.fallback, [False, True])
+    np.testing.assert_array_equal(cmp.warm_steps, [1000, 2500])   # case 1 kept its cold twin
+    np.testing.assert_array_equal(cmp.twin_steps, [500, 500])
+    per = 2.0 / 10000
+    np.testing.assert_allclose(cmp.warm_case_seconds - cmp.inference_seconds / 2,
+                               [(1000 + 500) * per, (2500 + 500) * per])
