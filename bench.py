@@ -209,6 +209,32 @@ def rb_batch(steps: int = 50000):
             "nusselt_256x130_400k_steps": nu_check}
 
 
+def reference_published_d2q9(calls: int = 3, per_call: int = 1000):
+    """The workload of the reference's only published rate (docs/source/guides/script.rst:
+    148-155): D2Q9 BGK on 2.0 M cells (104 MB at 52 B per cell), timed as run_sim does --
+    iterate(1000) including the host read-back of rho and u, MLUPS per batch of 1000
+    iterations (simulation.py:235-246).  The domain shape is not stated there; 2000 x 1000
+    periodic cells with a small shear flow stand in for it.  Also reports the kernel rate."""
+    import boltzmann_amd
+    from boltzmann_amd import core
+    nx, ny = 2000, 1000
+    sim = boltzmann_amd.Simulation("gpu", [nx, ny], core.bgk(0.6))
+    y = np.arange(ny)
+    sim.fluid.vel[..., 0] = (0.05 * np.sin(2 * np.pi * y / ny)).astype(np.float32)[None, :]
+    sim.iterate(2)
+    rates = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        sim.iterate(per_call)
+        rates.append(nx * ny * per_call / (1e6 * (time.perf_counter() - t0)))
+    ms = sim.step_timed(per_call)
+    out = {"cells": nx * ny, "run_sim_mlups_per_batch": [round(r, 1) for r in rates],
+           "kernel_mlups": round(nx * ny * per_call / (1e6 * ms * 1e-3), 1),
+           "reference_published_mlups": [1311, 1633, 1599, 1628, 1627]}
+    del sim
+    return out
+
+
 def cylinder_flow(nx=4096, ny=1024, r=64, u0=0.05):
     """Flags and initial velocity of the channel: fixed-velocity inflow column, fixed-density
     outflow column, walls top and bottom and the disc (x-1024)^2 + (y-512)^2 < r^2 (scaled
@@ -536,6 +562,9 @@ def main() -> int:
             variants[f"{coll}_{st}"] = v
             print(f"[bench] {coll} {st}: {r['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         variants["trt_f64"] = {"mlups": round(ref64["mlups"], 1)}
+        variants["d2q9_bgk_2M_reference_published_workload"] = reference_published_d2q9()
+        print(f"[bench] reference's published workload: "
+              f"{variants['d2q9_bgk_2M_reference_published_workload']}", file=sys.stderr, flush=True)
         variants["cylinder_d2q9_trt_4096x1024"] = cylinder()
         print("[bench] cylinder done", file=sys.stderr, flush=True)
         variants["rb_batch"] = rb_batch(args.rb_steps)
