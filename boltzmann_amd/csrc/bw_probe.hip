@@ -27,13 +27,18 @@
 //                 shift27_w4 and shift27_w4h with 2 cells per lane (8-byte accesses) on
 //                 rows of 128 cells
 // (each in-place figure the better of plain and streaming loads) and prints one JSON
-// object of GB/s (bytes read + written per second).  Every index is
+// object of GB/s (bytes read + written per second).
+//
+// `bw_probe REPS layout` measures instead how the plane layout moves the inplace27 and
+// shift27 ceilings: planes 2^26 bytes apart (the unpadded 256^3 lattice) against planes
+// padded by 256 B to 1 MiB, and the 27 planes interleaved in blocks of 64-1024 cells.  Every index is
 // checked against the element count it addresses.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #define CK(x)                                                                              \
     do {                                                                                   \
@@ -68,24 +73,42 @@ __global__ void write_planes(float* __restrict__ f, long long n) {
 }
 
 // NT: streaming (non-temporal) loads, as the update kernels use
+// Plane q starts q * st elements in (st >= n: st = n is the unpadded layout).
 template <int Q, typename T, bool NT>
-__global__ void inplace_planes(T* __restrict__ f, long long n) {
+__global__ void inplace_planes(T* __restrict__ f, long long n, long long st) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     T v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        if constexpr (NT) v[q] = __builtin_nontemporal_load(f + q * n + i);
-        else v[q] = f[q * n + i];
+        if constexpr (NT) v[q] = __builtin_nontemporal_load(f + q * st + i);
+        else v[q] = f[q * st + i];
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q * n + i] = v[(q + 1) % Q];
+    for (int q = 0; q < Q; ++q) f[q * st + i] = v[(q + 1) % Q];
+}
+
+// The same mix with the planes interleaved in blocks of B cells: block b holds Q runs of B
+// elements, population q of cell b * B + j at b * Q * B + q * B + j (n a multiple of B).
+template <int Q, int B, bool NT>
+__global__ void inplace_blocked(float* __restrict__ f, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* base = f + (i / B) * (Q * B) + (i % B);
+    float v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (NT) v[q] = __builtin_nontemporal_load(base + q * B);
+        else v[q] = base[q * B];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) base[q * B] = v[(q + 1) % Q];
 }
 
 // Each element (q, j) is read and written by exactly one thread: for a fixed q the map
 // i -> (i + shift_q) mod n is a bijection.  n is a power of two, so the wrap is a mask.
 template <typename T, bool NT>
-__global__ void shifted_planes(T* __restrict__ f, long long n) {
+__global__ void shifted_planes(T* __restrict__ f, long long n, long long st) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     T v[27];
@@ -93,7 +116,7 @@ __global__ void shifted_planes(T* __restrict__ f, long long n) {
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         const long long d = (q % 3 - 1) + 256 * ((q / 3) % 3 - 1) + 65536 * (q / 9 - 1);
-        a[q] = q * n + ((i + d) & (n - 1));
+        a[q] = q * st + ((i + d) & (n - 1));
         if constexpr (NT) v[q] = __builtin_nontemporal_load(f + a[q]);
         else v[q] = f[a[q]];
     }
@@ -279,28 +302,63 @@ int main(int argc, char** argv) {
     auto grid = [&](long long items) { return dim3((unsigned)((items + b - 1) / b)); };
     const long long n4 = total / 4;  // total is a multiple of 4
     const double lat = 2.0 * 4.0 * (double)total;
+    auto best = [](double a, double c) { return a > c ? a : c; };
+
+    if (argc > 2 && !strcmp(argv[2], "layout")) {
+        // inplace27 and shift27 with the planes st = n + pad floats apart (the unpadded
+        // stride is 2^26 bytes), and inplace27 with the planes interleaved in blocks.
+        // 26 * st + n <= 2 * total floats holds for every pad below.
+        const long long pads[] = {0, 64, 576, 1088, 16448, 262208};
+        printf("{\"pad_bytes\": [");
+        for (int k = 0; k < 6; ++k) printf("%s%lld", k ? ", " : "", pads[k] * 4);
+        printf("], \"inplace27\": [");
+        for (int k = 0; k < 6; ++k) {
+            const long long st = n + pads[k];
+            double r = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n, st); }, lat, reps),
+                            gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n, st); }, lat, reps));
+            printf("%s%.1f", k ? ", " : "", r);
+            fflush(stdout);
+        }
+        printf("], \"shift27\": [");
+        for (int k = 0; k < 6; ++k) {
+            const long long st = n + pads[k];
+            double r = best(gbs([&] { shifted_planes<float, false><<<grid(n), b>>>(f, n, st); }, lat, reps),
+                            gbs([&] { shifted_planes<float, true><<<grid(n), b>>>(f, n, st); }, lat, reps));
+            printf("%s%.1f", k ? ", " : "", r);
+            fflush(stdout);
+        }
+        double r64 = best(gbs([&] { inplace_blocked<27, 64, false><<<grid(n), b>>>(f, n); }, lat, reps),
+                          gbs([&] { inplace_blocked<27, 64, true><<<grid(n), b>>>(f, n); }, lat, reps));
+        double r256 = best(gbs([&] { inplace_blocked<27, 256, false><<<grid(n), b>>>(f, n); }, lat, reps),
+                           gbs([&] { inplace_blocked<27, 256, true><<<grid(n), b>>>(f, n); }, lat, reps));
+        double r1k = best(gbs([&] { inplace_blocked<27, 1024, false><<<grid(n), b>>>(f, n); }, lat, reps),
+                          gbs([&] { inplace_blocked<27, 1024, true><<<grid(n), b>>>(f, n); }, lat, reps));
+        printf("], \"inplace27_blocked\": {\"64\": %.1f, \"256\": %.1f, \"1024\": %.1f}}\n", r64, r256, r1k);
+        CK(hipFree(f));
+        CK(hipFree(sink));
+        return 0;
+    }
 
     double r_copy = gbs([&] { copy4<<<grid(n4), b>>>((const float4*)f, (float4*)g, n4); }, lat, reps);
     double r_read = gbs([&] { read_planes<27><<<grid(n), b>>>(f, sink, n); }, 4.0 * total, reps);
     double r_write = gbs([&] { write_planes<27><<<grid(n), b>>>(f, n); }, 4.0 * total, reps);
     // the in-place mixes, each the better of plain and streaming loads
-    auto best = [](double a, double c) { return a > c ? a : c; };
-    double r_in = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n); }, lat, reps),
-                       gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n); }, lat, reps));
-    double r_sh = best(gbs([&] { shifted_planes<float, false><<<grid(n), b>>>(f, n); }, lat, reps),
-                       gbs([&] { shifted_planes<float, true><<<grid(n), b>>>(f, n); }, lat, reps));
+    double r_in = best(gbs([&] { inplace_planes<27, float, false><<<grid(n), b>>>(f, n, n); }, lat, reps),
+                       gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n, n); }, lat, reps));
+    double r_sh = best(gbs([&] { shifted_planes<float, false><<<grid(n), b>>>(f, n, n); }, lat, reps),
+                       gbs([&] { shifted_planes<float, true><<<grid(n), b>>>(f, n, n); }, lat, reps));
     // 2-byte elements: 27 planes of n halves occupy half of the first lattice
     const double half = 2.0 * 2.0 * (double)total;
     double r_inh = best(
-        gbs([&] { inplace_planes<27, uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps),
-        gbs([&] { inplace_planes<27, uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps));
+        gbs([&] { inplace_planes<27, uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps),
+        gbs([&] { inplace_planes<27, uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps));
     double r_shh = best(
-        gbs([&] { shifted_planes<uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps),
-        gbs([&] { shifted_planes<uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps));
+        gbs([&] { shifted_planes<uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps),
+        gbs([&] { shifted_planes<uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps));
     // 9 planes over 3n cells: the same 27n floats
     double r_in9 = best(
-        gbs([&] { inplace_planes<9, float, false><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps),
-        gbs([&] { inplace_planes<9, float, true><<<grid(3 * n), b>>>(f, 3 * n); }, lat, reps));
+        gbs([&] { inplace_planes<9, float, false><<<grid(3 * n), b>>>(f, 3 * n, 3 * n); }, lat, reps),
+        gbs([&] { inplace_planes<9, float, true><<<grid(3 * n), b>>>(f, 3 * n, 3 * n); }, lat, reps));
 
     // wide accesses (vectors of VW words per lane; the 2-byte lattice is half the words)
     const long long nw = n;  // words per float plane
