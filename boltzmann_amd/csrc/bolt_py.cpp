@@ -635,6 +635,11 @@ PYBIND11_MODULE(_boltzmann, m) {
             // cells per lane of the fluid step (0: chosen per lattice); results do not change
             check(bolt_lattice_set_vector_width(s.core->lat, vw), "set_vector_width");
         }, py::arg("vw"))
+        .def("set_row_staging", [](PySimulation& s, bool on) {
+            // even steps through the row-staged kernel where the lattice allows it (default
+            // on); results do not change
+            check(bolt_lattice_set_row_staging(s.core->lat, on ? 1 : 0), "set_row_staging");
+        }, py::arg("on"))
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;
             check(bolt_lattice_device_bytes(s.core->lat, &b), "device_bytes");

@@ -80,6 +80,7 @@ struct bolt_lattice {
     std::vector<Coupling> couplings;
     int block = 0;
     int vw = 0;                  // cells per lane of the fluid step (0: chosen per lattice)
+    bool row_even = true;        // even steps row-staged (lbm_step_row) where they qualify
     double* partials = nullptr;  // reduction scratch
     double* result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -143,6 +144,10 @@ static StepFn lookup_step(int d, int q, int coll, int s, bool even, int tq) {
 // wide fluid steps (update_wide.hpp, instantiated in bolt_wide.hip)
 using WideFn = void (*)(StepParams, WideGeom);
 WideFn bolt_lookup_wide(int d, int q, int coll, int s, int vw, bool even);
+
+// row-staged even steps (update.hpp lbm_step_row, instantiated in bolt_row.hip)
+StepFn bolt_lookup_row(int d, int q, int coll, int s, int block);
+int bolt_row_slots(int d, int q);
 
 static StepFn lookup_ad(int d, int tq, int s, bool even) {
     if (d == 2 && tq == 5) {
@@ -921,13 +926,26 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwe, L->storage);
     if (vwo > 1 && !(wo = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwo, L->storage);
+    // the even step row-staged: one work-group per row of the contiguous axis, which must
+    // fit one work-group (1024 cells) and its shifted planes' rows 64 KiB of LDS
+    StepFn fr = nullptr;
+    const int row_block = (p.len + 63) / 64 * 64;
+    const size_t row_lds = (size_t)bolt_row_slots(L->ndim, L->q) * row_block * store_size(L->storage);
+    if (L->row_even && !can_fuse && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
+        fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, row_block);
+    const long long rows = L->ncell / p.len;
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
     const dim3 wgride((unsigned)wge.nblocks), wgrido((unsigned)wgo.nblocks), wblk(256);
+    const dim3 rgrid((unsigned)rows), rblk((unsigned)row_block);
     hipStream_t s = L->ctx->stream;
     for (uint64_t it = 0; it < n; ++it) {
         const bool even = (it % 2) == 0;
         p.write_macro = (it + 1 == n) || sep_tracers;
-        if (even && vwe > 1) hipLaunchKernelGGL(we, wgride, wblk, 0, s, p, wge);
+        if (even && fr) {
+            StepParams pr = p;
+            pr.nblocks = (int)rows;
+            hipLaunchKernelGGL(fr, rgrid, rblk, row_lds, s, pr);
+        } else if (even && vwe > 1) hipLaunchKernelGGL(we, wgride, wblk, 0, s, p, wge);
         else if (!even && vwo > 1) hipLaunchKernelGGL(wo, wgrido, wblk, 0, s, p, wgo);
         else hipLaunchKernelGGL(even ? fe : fo, grid, blk, 0, s, p);
         for (size_t k = first_sep; k < L->tracers.size(); ++k) {
@@ -1066,6 +1084,12 @@ int bolt_lattice_set_vector_width(bolt_lattice* L, int vw) {
     if (vw > 1 && L->n[L->ndim - 1] % vw != 0)
         return fail(BOLT_EINVAL, "the contiguous axis (%d cells) is not a multiple of %d", L->n[L->ndim - 1], vw);
     L->vw = vw;
+    return BOLT_OK;
+}
+
+int bolt_lattice_set_row_staging(bolt_lattice* L, int on) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    L->row_even = on != 0;
     return BOLT_OK;
 }
 

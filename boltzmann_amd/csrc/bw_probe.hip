@@ -268,6 +268,63 @@ __global__ void __launch_bounds__(256) shift_wide(uint32_t* __restrict__ f, long
     }
 }
 
+// The neighbour mix with every access an aligned row: one 256-thread work-group per row of
+// 256 cells (rows are (x, y) pairs of a 256^3 lattice, the row axis contiguous), one cell per
+// thread.  Each plane is read from row (x + dx, y + dy) at the thread's own position; the 18
+// planes with a shift along the row go through LDS, where thread z takes element
+// (z + dz) mod 256 (the periodic neighbour inside the same row) and later puts its result
+// back at that element before the row is stored aligned.  Thread z owns element (z + dz) mod
+// 256 of each shifted plane, a bijection, so every element is read and written once.
+template <typename T>
+__global__ void __launch_bounds__(256) shift_lds(T* __restrict__ f, long long n) {
+    __shared__ T buf[18][256];
+    const int z = (int)threadIdx.x;
+    const int r = (int)blockIdx.x;  // n / 256 rows: x = r / 256, y = r % 256
+    const int x = r >> 8, y = r & 255;
+    T v[27];
+    long long a[27];
+#pragma unroll
+    for (int q = 0, k = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1, dy = (q / 3) % 3 - 1, dx = q / 9 - 1;
+        const long long rr = (((x + dx) & 255) << 8) | ((y + dy) & 255);
+        a[q] = q * n + rr * 256 + z;
+        const T e = __builtin_nontemporal_load(f + a[q]);
+        if (dz == 0) v[q] = e;
+        else buf[k++][z] = e;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0, k = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1;
+        if (dz != 0) v[q] = buf[k++][(z + dz) & 255];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0, k = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1;
+        if (dz != 0) buf[k++][(z + dz) & 255] = v[(q + 1) % 27];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0, k = 0; q < 27; ++q) {
+        const int dz = q % 3 - 1;
+        f[a[q]] = dz == 0 ? v[(q + 1) % 27] : buf[k++][z];
+    }
+}
+
+// inplace27 with the work-group's loads and stores in two phases: every thread of the
+// work-group has loaded its 27 elements before any of them stores (a barrier between).
+template <typename T>
+__global__ void __launch_bounds__(256) inplace_phased(T* __restrict__ f, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // n % 256 == 0
+    T v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = __builtin_nontemporal_load(f + q * n + i);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[q * n + i] = v[(q + 1) % 27];
+}
+
 template <typename F>
 static double gbs(F launch, double bytes, int reps) {
     hipEvent_t a, b;
@@ -302,6 +359,7 @@ int main(int argc, char** argv) {
     auto grid = [&](long long items) { return dim3((unsigned)((items + b - 1) / b)); };
     const long long n4 = total / 4;  // total is a multiple of 4
     const double lat = 2.0 * 4.0 * (double)total;
+    const double half = 2.0 * 2.0 * (double)total;
     auto best = [](double a, double c) { return a > c ? a : c; };
 
     if (argc > 2 && !strcmp(argv[2], "layout")) {
@@ -333,7 +391,26 @@ int main(int argc, char** argv) {
                            gbs([&] { inplace_blocked<27, 256, true><<<grid(n), b>>>(f, n); }, lat, reps));
         double r1k = best(gbs([&] { inplace_blocked<27, 1024, false><<<grid(n), b>>>(f, n); }, lat, reps),
                           gbs([&] { inplace_blocked<27, 1024, true><<<grid(n), b>>>(f, n); }, lat, reps));
-        printf("], \"inplace27_blocked\": {\"64\": %.1f, \"256\": %.1f, \"1024\": %.1f}}\n", r64, r256, r1k);
+        // the aligned-row neighbour mix through LDS, for 4-, 2- and 8-byte elements, beside
+        // the plain in-place and neighbour mixes of 8-byte elements (27 n doubles fill the
+        // buffer); 256 rows x 256 cells per row x 256 = n cells
+        const unsigned rows = (unsigned)(n / 256);
+        double rl = gbs([&] { shift_lds<float><<<rows, 256>>>(f, n); }, lat, reps);
+        double rlh = gbs([&] { shift_lds<uint16_t><<<rows, 256>>>((uint16_t*)f, n); }, half, reps);
+        const double dbl = 2.0 * 8.0 * (double)total;
+        double rld = gbs([&] { shift_lds<double><<<rows, 256>>>((double*)f, n); }, dbl, reps);
+        double rid = best(gbs([&] { inplace_planes<27, double, false><<<grid(n), b>>>((double*)f, n, n); }, dbl, reps),
+                          gbs([&] { inplace_planes<27, double, true><<<grid(n), b>>>((double*)f, n, n); }, dbl, reps));
+        double rsd = best(gbs([&] { shifted_planes<double, false><<<grid(n), b>>>((double*)f, n, n); }, dbl, reps),
+                          gbs([&] { shifted_planes<double, true><<<grid(n), b>>>((double*)f, n, n); }, dbl, reps));
+        double rp = gbs([&] { inplace_phased<float><<<grid(n), b>>>(f, n); }, lat, reps);
+        double rph = gbs([&] { inplace_phased<uint16_t><<<grid(n), b>>>((uint16_t*)f, n); }, half, reps);
+        double rpd = gbs([&] { inplace_phased<double><<<grid(n), b>>>((double*)f, n); }, dbl, reps);
+        printf("], \"inplace27_blocked\": {\"64\": %.1f, \"256\": %.1f, \"1024\": %.1f}, "
+               "\"shift27_lds\": %.1f, \"shift27_lds_h\": %.1f, \"shift27_lds_d\": %.1f, "
+               "\"inplace27_d\": %.1f, \"shift27_d\": %.1f, \"inplace27_phased\": %.1f, "
+               "\"inplace27_phased_h\": %.1f, \"inplace27_phased_d\": %.1f}\n",
+               r64, r256, r1k, rl, rlh, rld, rid, rsd, rp, rph, rpd);
         CK(hipFree(f));
         CK(hipFree(sink));
         return 0;
@@ -347,8 +424,7 @@ int main(int argc, char** argv) {
                        gbs([&] { inplace_planes<27, float, true><<<grid(n), b>>>(f, n, n); }, lat, reps));
     double r_sh = best(gbs([&] { shifted_planes<float, false><<<grid(n), b>>>(f, n, n); }, lat, reps),
                        gbs([&] { shifted_planes<float, true><<<grid(n), b>>>(f, n, n); }, lat, reps));
-    // 2-byte elements: 27 planes of n halves occupy half of the first lattice
-    const double half = 2.0 * 2.0 * (double)total;
+    // 2-byte elements: 27 planes of n halves occupy half of the first lattice (half: bytes)
     double r_inh = best(
         gbs([&] { inplace_planes<27, uint16_t, false><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps),
         gbs([&] { inplace_planes<27, uint16_t, true><<<grid(n), b>>>((uint16_t*)f, n, n); }, half, reps));

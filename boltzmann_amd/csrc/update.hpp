@@ -94,6 +94,8 @@ struct Nbr<2> {
         y[2] = iy == p.n1 - 1 ? 0 : iy + 1;
     }
     __device__ uint32_t at(int dx, int dy, int) const { return xb[dx + 1] + y[dy + 1]; }
+    // first cell of the row at (ix + dx); the row is the contiguous axis
+    __device__ uint32_t row_at(int dx, int) const { return xb[dx + 1]; }
 };
 
 template <>
@@ -114,6 +116,8 @@ struct Nbr<3> {
         z[2] = iz == p.n2 - 1 ? 0 : iz + 1;
     }
     __device__ uint32_t at(int dx, int dy, int dz) const { return rb[dx + 1][dy + 1] + z[dz + 1]; }
+    // first cell of the row at (ix + dx, iy + dy); the row is the contiguous axis
+    __device__ uint32_t row_at(int dx, int dy) const { return rb[dx + 1][dy + 1]; }
 };
 
 // ---- AA loads and stores ---------------------------------------------------------------
@@ -139,28 +143,43 @@ __device__ inline void pop_store(void* plane, uint32_t cell, T v) {
 #endif
 }
 
-template <class V, bool EVEN, class T, class NB>
-__device__ inline void aa_load_raw(void* const* planes, const NB& nb, uint32_t cell,
-                                   T (&v)[V::q]) {
-#pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        if (EVEN)
-            v[i] = pop_load<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)));
-        else
-            v[i] = pop_load<T>(planes[V::opp(i)], cell);
+// An accessor reaches the AA locations of one cell by position k: slot k of plane k at
+// x - c_k on the even step, at x on the odd step.  ld<V, T>(k) loads position k and
+// st<V, T>(k, v) stores it.  CellAA addresses every position in memory (one cell per
+// work-item); the row-staged even step (RowAA, lbm_step_row) keeps the positions shifted
+// along the row in LDS.
+template <int D, bool EVEN>
+struct CellAA {
+    void* const* planes;
+    const Nbr<D>& nb;
+    uint32_t cell;
+    template <class V>
+    __device__ uint32_t at(int k) const {
+        return EVEN ? nb.at(-V::c(k, 0), -V::c(k, 1), -V::c(k, 2)) : cell;
     }
+    template <class V, class T>
+    __device__ T ld(int k) const {
+        return pop_load<T>(planes[k], at<V>(k));
+    }
+    template <class V, class T>
+    __device__ void st(int k, T v) const {
+        pop_store<T>(planes[k], at<V>(k), v);
+    }
+};
+
+// f_i of the cell: position i on the even step, position opp(i) on the odd step.
+template <class V, bool EVEN, class T, class A>
+__device__ inline void aa_load_raw(const A& a, T (&v)[V::q]) {
+#pragma unroll
+    for (int i = 0; i < V::q; ++i) v[i] = a.template ld<V, T>(EVEN ? i : V::opp(i));
 }
 
-template <class V, bool EVEN, class T, class NB>
-__device__ inline void aa_store_raw(void* const* planes, const NB& nb, uint32_t cell,
-                                    const T (&v)[V::q]) {
+// The post-collision populations: position i takes f*_opp(i) on the even step, f*_i on
+// the odd step.
+template <class V, bool EVEN, class T, class A>
+__device__ inline void aa_store_raw(const A& a, const T (&v)[V::q]) {
 #pragma unroll
-    for (int i = 0; i < V::q; ++i) {
-        if (EVEN)
-            pop_store<T>(planes[i], nb.at(-V::c(i, 0), -V::c(i, 1), -V::c(i, 2)), v[V::opp(i)]);
-        else
-            pop_store<T>(planes[i], cell, v[i]);
-    }
+    for (int i = 0; i < V::q; ++i) a.template st<V, T>(i, EVEN ? v[V::opp(i)] : v[i]);
 }
 
 // ---- reference arithmetic (storage kF32) -------------------------------------------------
@@ -283,24 +302,22 @@ __device__ inline void ref_pair(float (&f)[V::q], const float (&u)[3], float vv,
 
 // The reference-arithmetic collision fused with the AA stores, one opposite pair at a time,
 // so each population's register is free as soon as it is stored.
-template <class V, bool EVEN, int COLL, class NB, int I = 0>
+template <class V, bool EVEN, int COLL, class A, int I = 0>
 __device__ inline void ref_collide_store(float (&f)[V::q], const float (&u)[3], float vv, float r,
-                                         float omp, float omn, void* const* planes, const NB& nb,
-                                         uint32_t cell) {
+                                         float omp, float omn, const A& a) {
     if constexpr (I < V::q) {
         constexpr int J = V::opp(I);
         if constexpr (J >= I) {
             ref_pair<V, COLL, I>(f, u, vv, r, omp, omn);
             if constexpr (EVEN) {
-                pop_store<float>(planes[I], nb.at(-V::c(I, 0), -V::c(I, 1), -V::c(I, 2)), f[J]);
-                if constexpr (J != I)
-                    pop_store<float>(planes[J], nb.at(-V::c(J, 0), -V::c(J, 1), -V::c(J, 2)), f[I]);
+                a.template st<V, float>(I, f[J]);
+                if constexpr (J != I) a.template st<V, float>(J, f[I]);
             } else {
-                pop_store<float>(planes[I], cell, f[I]);
-                if constexpr (J != I) pop_store<float>(planes[J], cell, f[J]);
+                a.template st<V, float>(I, f[I]);
+                if constexpr (J != I) a.template st<V, float>(J, f[J]);
             }
         }
-        ref_collide_store<V, EVEN, COLL, NB, I + 1>(f, u, vv, r, omp, omn, planes, nb, cell);
+        ref_collide_store<V, EVEN, COLL, A, I + 1>(f, u, vv, r, omp, omn, a);
     }
 }
 
@@ -591,20 +608,19 @@ constexpr int waves_target() {
 #endif
 }
 
-// One cell of one time step: fluid update, optionally followed by one tracer (TQ != 0).
-template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
-__device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, uint32_t cell) {
+// One non-wall cell of one time step: fluid update through the accessor `a` (its AA
+// locations), optionally followed by one tracer (TQ != 0, addressed through nb).
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ, class A>
+__device__ inline void update_cell_with(const StepParams& p, const A& a, const Nbr<D>& nb, int flag,
+                                        uint32_t cell) {
     using V = VSet<D, Q>;
     using ST = Store<S>;
     using T = typename ST::T;
-    const int flag = elem<uint8_t>((const void*)p.flags, cell);
-    if (flag & 1) return;
     const bool fixed = (flag & 6) != 0;
-    const Nbr<D> nb(ix, iy, iz, p);
 
     if constexpr (S == kF64) {
         double f[Q];
-        aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
+        aa_load_raw<V, EVEN>(a, f);
         double rfix = 0.0, ufix[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
         if (fixed) {
             rfix = elem<double>((const void*)p.rho, cell);
@@ -622,7 +638,7 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
         }
         double r, u[3];
         f64_step<V, COLL>(f, r, u, fixed, p, rfix, ufix, acc, use_acc);
-        aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
+        aa_store_raw<V, EVEN>(a, f);
         if (p.write_macro && !fixed) {
             elem<double>(p.rho, cell) = r;
 #pragma unroll
@@ -631,12 +647,12 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
         if constexpr (TQ != 0) {
             using W = VSet<D, TQ>;
             double g[TQ];
-            aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
+            aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
             const bool tfixed = (flag & 8) != 0;
             double rt;
             f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
                       tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
-            aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+            aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
             elem<double>(p.tC, cell) = rt;
         }
     } else {
@@ -644,16 +660,16 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
         float r, u[3] = {0.f, 0.f, 0.f};
         float omp = p.om_p, omn = p.om_n;
         if constexpr (S == kF32) {
-            aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
+            aa_load_raw<V, EVEN>(a, f);
             ref_moments<V>(f, r, u);
         } else if constexpr (S == kF32F) {
-            aa_load_raw<V, EVEN>(p.fpl, nb, cell, f);
+            aa_load_raw<V, EVEN>(a, f);
 #pragma unroll
             for (int i = 0; i < Q; ++i) f[i] -= V::w(i);
             dev_moments<V>(f, r, u);  // r holds rho - 1 here
         } else {
             T raw[Q];
-            aa_load_raw<V, EVEN>(p.fpl, nb, cell, raw);
+            aa_load_raw<V, EVEN>(a, raw);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 if constexpr (S == kF16S) f[i] = (float)raw[i];
@@ -684,12 +700,12 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
         }
         if constexpr (S == kF32) {
             const float vv = ref_vv<D>(u);
-            ref_collide_store<V, EVEN, COLL>(f, u, vv, r, omp, omn, p.fpl, nb, cell);
+            ref_collide_store<V, EVEN, COLL>(f, u, vv, r, omp, omn, a);
         } else if constexpr (S == kF32F) {
             dev_collide<V, COLL>(f, u, dev_vv(u), r, omp, omn);
 #pragma unroll
             for (int i = 0; i < Q; ++i) f[i] += V::w(i);
-            aa_store_raw<V, EVEN>(p.fpl, nb, cell, f);
+            aa_store_raw<V, EVEN>(a, f);
             r = r + 1.0f;
         } else {
             dev_collide<V, COLL>(f, u, dev_vv(u), r, omp, omn);
@@ -699,7 +715,7 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
                 if constexpr (S == kF16S) raw[i] = (_Float16)f[i];
                 else raw[i] = f32_to_bf16_rne(f[i]);
             }
-            aa_store_raw<V, EVEN>(p.fpl, nb, cell, raw);
+            aa_store_raw<V, EVEN>(a, raw);
             r = r + 1.0f;
         }
         if (p.write_macro && !fixed) {
@@ -711,15 +727,24 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
             static_assert(S == kF32, "fused tracers use float storage");
             using W = VSet<D, TQ>;
             float g[TQ];
-            aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
+            aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
             const bool tfixed = (flag & 8) != 0;
             float rt;
             ref_ad<W>(g, rt, u, tfixed, p.t_om,
                       tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
-            aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+            aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
             elem<float>(p.tC, cell) = rt;
         }
     }
+}
+
+// One cell of one time step, every AA location addressed in memory (one cell per work-item).
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
+__device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, uint32_t cell) {
+    const int flag = elem<uint8_t>((const void*)p.flags, cell);
+    if (flag & 1) return;
+    const Nbr<D> nb(ix, iy, iz, p);
+    update_cell_with<D, Q, COLL, S, EVEN, TQ>(p, CellAA<D, EVEN>{p.fpl, nb, cell}, nb, flag, cell);
 }
 
 template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
@@ -730,6 +755,110 @@ __global__ void __launch_bounds__(256)
     uint32_t cell;
     if (!locate<D>(p, ix, iy, iz, cell)) return;
     update_cell<D, Q, COLL, S, EVEN, TQ>(p, ix, iy, iz, cell);
+}
+
+// ---- row-staged even step ----------------------------------------------------------------
+// The even (streaming) step with every global access an aligned row.  One work-group owns a
+// whole row of the contiguous axis (len <= blockDim.x <= 1024 cells, one per work-item).
+// Position k of cell z lies in row (x - c_k) of plane k, at element z - c_k,row of that row
+// (c_k,row: the component along the contiguous axis; the lattice is periodic, so the
+// element wraps within the row).  Work-item z loads element z of every plane's row: where
+// c_k,row = 0 that is its own position (kept in registers); the other planes' rows go to
+// LDS, where work-item z reads and later writes element (z - c_k,row) mod len -- a
+// bijection, so each element has exactly one owner and needs no barrier between its read
+// and its write.  After a barrier the shifted planes' rows are stored from LDS at element
+// z.  Elements owned by wall cells, which neither read nor write, go back as loaded.  The
+// per-cell arithmetic is update_cell_with, shared with lbm_step, so both kernels give
+// identical bits.
+
+// LDS row of plane k (planes with no component along the contiguous axis have none).
+template <class V>
+__host__ __device__ constexpr int row_slot(int k) {
+    int s = 0;
+    for (int j = 0; j < k; ++j) s += V::c(j, V::d - 1) != 0;
+    return s;
+}
+template <class V>
+__host__ __device__ constexpr int row_slots() {
+    return row_slot<V>(V::q);
+}
+
+template <int D, class T>
+struct RowAA {
+    void* const* planes;
+    const Nbr<D>& nb;
+    const T* own;  // element z of the rows of the planes with c_k,row = 0
+    T* lds;        // rows of the other planes: row_slot(k) * stride + element
+    int stride, z, len;
+    template <class V>
+    __device__ int lds_index(int k) const {
+        const int e = z - V::c(k, D - 1);
+        return row_slot<V>(k) * stride + (e < 0 ? e + len : (e >= len ? e - len : e));
+    }
+    template <class V, class U>
+    __device__ U ld(int k) const {
+        static_assert(sizeof(U) == sizeof(T), "storage element");
+        if (V::c(k, D - 1) == 0) return own[k];
+        return lds[lds_index<V>(k)];
+    }
+    template <class V, class U>
+    __device__ void st(int k, U v) const {
+        static_assert(sizeof(U) == sizeof(T), "storage element");
+        if (V::c(k, D - 1) == 0)
+            pop_store<U>(planes[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z, v);
+        else
+            lds[lds_index<V>(k)] = v;
+    }
+};
+
+// BMAX: the largest work-group the instantiation serves (256 or 1024).  Launched with
+// p.nblocks = rows, blockDim.x = len rounded up to 64 and row_slots * blockDim.x elements
+// of dynamic LDS.
+template <int D, int Q, int COLL, int S, int BMAX>
+__global__ void __launch_bounds__(BMAX)
+    __attribute__((amdgpu_waves_per_eu(BMAX > 256 ? BMAX / 256 : waves_target<Q, S, 0>())))
+    lbm_step_row(const StepParams p) {
+    using V = VSet<D, Q>;
+    using T = typename Store<S>::T;
+    extern __shared__ __align__(16) unsigned char row_lds_bytes[];
+    T* lds = (T*)row_lds_bytes;
+    const int row = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    const int z = (int)threadIdx.x;
+    const int stride = (int)blockDim.x;
+    const bool live = z < p.len;
+    int ix, iy;
+    if (D == 2) {
+        ix = row;
+        iy = z;
+    } else {
+        ix = row / p.n1;
+        iy = row - ix * p.n1;
+    }
+    const Nbr<D> nb(ix, iy, z, p);
+    const uint32_t cell = nb.row_at(0, 0) + (uint32_t)z;
+    int flag = 1;
+    T own[Q];
+    if (live) {
+        flag = elem<uint8_t>((const void*)p.flags, cell);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const T v = pop_load<T>(p.fpl[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z);
+            if (V::c(k, D - 1) == 0) own[k] = v;
+            else lds[row_slot<V>(k) * stride + z] = v;
+        }
+    }
+    __syncthreads();
+    if (live && !(flag & 1))
+        update_cell_with<D, Q, COLL, S, true, 0>(p, RowAA<D, T>{p.fpl, nb, own, lds, stride, z, p.len}, nb,
+                                                 flag, cell);
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            if (V::c(k, D - 1) != 0)
+                pop_store<T>(p.fpl[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z,
+                             lds[row_slot<V>(k) * stride + z]);
+    }
 }
 
 // Advection-diffusion tracer on its own, reading the fluid velocity written this step.
@@ -745,25 +874,25 @@ __global__ void __launch_bounds__(256) ad_step(const StepParams p) {
     const Nbr<D> nb(ix, iy, iz, p);
     if constexpr (S == kF64) {
         double g[TQ];
-        aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
+        aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
         double u[3] = {0, 0, 0};
 #pragma unroll
         for (int a = 0; a < D; ++a) u[a] = elem<double>((const void*)p.vel[a], cell);
         double rt;
         f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
                   tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
-        aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+        aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
         elem<double>(p.tC, cell) = rt;
     } else {
         float g[TQ];
-        aa_load_raw<W, EVEN>(p.gpl, nb, cell, g);
+        aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
         float u[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int a = 0; a < D; ++a) u[a] = elem<float>((const void*)p.vel[a], cell);
         float rt;
         ref_ad<W>(g, rt, u, tfixed, p.t_om,
                   tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
-        aa_store_raw<W, EVEN>(p.gpl, nb, cell, g);
+        aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
         elem<float>(p.tC, cell) = rt;
     }
 }

@@ -189,8 +189,8 @@ int bolt_lattice_postproc(bolt_lattice* lat, int kind, const int* origin, float*
 /* Device bytes held by the lattice's buffers. */
 int bolt_lattice_device_bytes(const bolt_lattice* lat, uint64_t* out);
 
-/* Launch tuning: work-group size along the contiguous axis (64..1024, multiple of 64);
- * 0 restores the built-in choice. */
+/* Launch tuning: work-group size along the contiguous axis (64, 128, 192 or 256) of the
+ * one-cell-per-lane steps; 0 restores the built-in choice. */
 int bolt_lattice_set_block(bolt_lattice* lat, int block);
 /* Launch tuning: cells of the contiguous axis per lane in the fluid step (1, 2 or 4; 2 at
  * most for double storage; the axis must be a multiple of it), on every step; 0 restores
@@ -200,6 +200,15 @@ int bolt_lattice_set_block(bolt_lattice* lat, int block);
  * the row with lane rotations.  A tracer fused into the fluid step always runs one cell per
  * lane.  Results do not depend on the choice (bit-identical). */
 int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
+/* Launch tuning (no counterpart in the reference, whose kernels take one cell per
+ * work-item): on (the default) runs the even (streaming) step row-staged where the lattice
+ * allows it -- one work-group per row of the contiguous axis, every population access an
+ * aligned row, the shift along the row taken through LDS.  It applies when one cell per
+ * lane is chosen for the even step, no tracer is fused into the fluid step, the row has at
+ * most 1024 cells and its shifted planes' rows fit 64 KiB of LDS; otherwise, and when off,
+ * the even step addresses each neighbour's slot directly.  Results do not depend on the
+ * choice (bit-identical). */
+int bolt_lattice_set_row_staging(bolt_lattice* lat, int on);
 
 /* ---- batches: many small 2D Rayleigh-Benard-type lattices in one launch per step -----
  * No counterpart in the reference (it runs one Simulation per process).  Instance k is a
