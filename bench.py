@@ -430,8 +430,9 @@ def profile_kernels(steps: int = 20) -> dict:
     """A short run of every hot-path kernel for a profiler: the 256^3 D3Q27 TRT lattice in
     every storage format, the 4096 x 1024 D2Q9 TRT cylinder channel in f32 and f16s, a
     1024 x 1024 D2Q9 lattice with a fused D2Q5 tracer and Boussinesq coupling, and the
-    64-cell Rayleigh-Benard batch with direct launches (no graph replay).  Returns the
-    launches and the algorithmic bytes per launch of each workload."""
+    64-cell Rayleigh-Benard batch with direct launches (no graph replay), its even steps
+    row-staged and then direct (the autotuner's two choices).  Returns the launches and the
+    algorithmic bytes per launch of each workload."""
     import boltzmann_amd
     from boltzmann_amd import Omega, core, rb
     out = {}
@@ -463,12 +464,15 @@ def profile_kernels(steps: int = 20) -> dict:
     sim.step_timed(10 * steps)
     out["fused_1024x1024_d2q9_trt_d2q5"] = {"launches": 10 * steps + 2, "bytes_per_launch": m * m * 121}
     del sim
-    b = rb.make_batch(rb.sweep(64, seed=4))
-    b.set_shape(128, 0)
-    b.step(10 * steps)
-    b.step_timed(2)
-    out["rb_batch_64"] = {"launches": 10 * steps + 2, "bytes_per_launch": int(b.cells) * 121}
-    del b
+    for staged in (True, False):
+        b = rb.make_batch(rb.sweep(64, seed=4))
+        b.set_shape(128, 0)
+        b.set_row_staging(staged)
+        b.step(10 * steps)
+        b.step_timed(2)
+        out[f"rb_batch_64_rows_{'staged' if staged else 'direct'}"] = {
+            "launches": 10 * steps + 2, "bytes_per_launch": int(b.cells) * 121}
+        del b
     return out
 
 

@@ -752,6 +752,9 @@ PYBIND11_MODULE(_boltzmann, m) {
         .def("set_shape", [](PyBatch& b, int block, int graph_steps) {
             check(bolt_batch_set_shape(b.core->batch, block, graph_steps), "set_shape");
         }, py::arg("block"), py::arg("graph_steps"))
+        .def("set_row_staging", [](PyBatch& b, bool on) {
+            check(bolt_batch_set_row_staging(b.core->batch, on ? 1 : 0), "set_row_staging");
+        }, py::arg("on"))
         .def("set_active", [](PyBatch& b, py::array_t<bool, py::array::c_style | py::array::forcecast> a) {
             std::vector<uint8_t> v(a.data(), a.data() + a.size());
             check(bolt_batch_set_active(b.core->batch, v.data(), (int)v.size()), "set_active");

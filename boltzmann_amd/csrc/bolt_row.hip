@@ -1,9 +1,9 @@
 // The row-staged even steps of libbolt_hip.so (update.hpp lbm_step_row: one work-group per
 // row of the contiguous axis, every population access an aligned row, the shift along the
-// row taken in LDS; no fused tracer), in a translation unit of their own so that they
-// compile beside bolt_runtime.hip.  bolt_lookup_row returns the kernel for a (D, Q,
-// collision, storage) and a work-group of `block` threads (block <= 256, or <= 1024 where
-// an instantiation for long rows exists), or null.
+// row taken in LDS; with or without a fused tracer), in a translation unit of their own so
+// that they compile beside bolt_runtime.hip.  bolt_lookup_row returns the kernel for a (D,
+// Q, collision, storage, fused tracer's Q or 0) and a work-group of `block` threads (block
+// <= 256, or <= 1024 where an instantiation for long rows exists), or null.
 #include <hip/hip_runtime.h>
 
 #include "update.hpp"
@@ -12,37 +12,51 @@ using namespace bolt;
 
 using StepFn = void (*)(StepParams);
 
-template <int D, int Q, int C, int S>
+template <int D, int Q, int C, int S, int TQ>
 static StepFn row_fn(int block) {
-    if (block <= 256) return (StepFn)lbm_step_row<D, Q, C, S, 256>;
+    if (block <= 256) return (StepFn)lbm_step_row<D, Q, C, S, 256, TQ>;
     // long rows: D2Q9 in every storage, D3Q27 in the formats whose registers fit 1024
-    // work-items (double storage does not)
-    if constexpr (Q == 9 || S != kF64) return (StepFn)lbm_step_row<D, Q, C, S, 1024>;
+    // work-items (double storage and fused tracers do not)
+    if constexpr (Q == 9 || (S != kF64 && TQ == 0)) return (StepFn)lbm_step_row<D, Q, C, S, 1024, TQ>;
     return nullptr;
 }
 
-template <int D, int Q, int C>
-static StepFn row_fn_s(int s, int block) {
-    switch (s) {
-        case kF32: return row_fn<D, Q, C, kF32>(block);
-        case kF32F: return row_fn<D, Q, C, kF32F>(block);
-        case kF16S: return row_fn<D, Q, C, kF16S>(block);
-        case kBF16S: return row_fn<D, Q, C, kBF16S>(block);
-        case kF64: return row_fn<D, Q, C, kF64>(block);
+template <int D, int Q, int C, int S>
+static StepFn row_fn_t(int tq, int block) {
+    if (tq == 0) return row_fn<D, Q, C, S, 0>(block);
+    if constexpr (S == kF32 || S == kF64) {
+        constexpr int TQ = D == 2 ? 5 : 7;
+        if (tq == TQ) return row_fn<D, Q, C, S, TQ>(block);
     }
     return nullptr;
 }
 
-StepFn bolt_lookup_row(int d, int q, int coll, int s, int block) {
-    if (block < 64 || block > 1024 || block % 64 != 0) return nullptr;
-    if (d == 2 && q == 9) return coll == 0 ? row_fn_s<2, 9, 0>(s, block) : row_fn_s<2, 9, 1>(s, block);
-    if (d == 3 && q == 27) return coll == 0 ? row_fn_s<3, 27, 0>(s, block) : row_fn_s<3, 27, 1>(s, block);
+template <int D, int Q, int C>
+static StepFn row_fn_s(int s, int tq, int block) {
+    switch (s) {
+        case kF32: return row_fn_t<D, Q, C, kF32>(tq, block);
+        case kF32F: return row_fn_t<D, Q, C, kF32F>(tq, block);
+        case kF16S: return row_fn_t<D, Q, C, kF16S>(tq, block);
+        case kBF16S: return row_fn_t<D, Q, C, kBF16S>(tq, block);
+        case kF64: return row_fn_t<D, Q, C, kF64>(tq, block);
+    }
     return nullptr;
 }
 
-// LDS elements one work-group of the row-staged step needs per cell of its row.
+StepFn bolt_lookup_row(int d, int q, int coll, int s, int tq, int block) {
+    if (block < 64 || block > 1024 || block % 64 != 0) return nullptr;
+    if (d == 2 && q == 9) return coll == 0 ? row_fn_s<2, 9, 0>(s, tq, block) : row_fn_s<2, 9, 1>(s, tq, block);
+    if (d == 3 && q == 27)
+        return coll == 0 ? row_fn_s<3, 27, 0>(s, tq, block) : row_fn_s<3, 27, 1>(s, tq, block);
+    return nullptr;
+}
+
+// LDS elements one work-group of the row-staged step needs per cell of its row, for a
+// fluid (q = 9, 27) or a tracer (q = 5, 7) velocity set; 0 for q = 0.
 int bolt_row_slots(int d, int q) {
     if (d == 2 && q == 9) return row_slots<VSet<2, 9>>();
     if (d == 3 && q == 27) return row_slots<VSet<3, 27>>();
+    if (d == 2 && q == 5) return row_slots<VSet<2, 5>>();
+    if (d == 3 && q == 7) return row_slots<VSet<3, 7>>();
     return 0;
 }

@@ -146,7 +146,7 @@ using WideFn = void (*)(StepParams, WideGeom);
 WideFn bolt_lookup_wide(int d, int q, int coll, int s, int vw, bool even);
 
 // row-staged even steps (update.hpp lbm_step_row, instantiated in bolt_row.hip)
-StepFn bolt_lookup_row(int d, int q, int coll, int s, int block);
+StepFn bolt_lookup_row(int d, int q, int coll, int s, int tq, int block);
 int bolt_row_slots(int d, int q);
 
 static StepFn lookup_ad(int d, int tq, int s, bool even) {
@@ -927,12 +927,14 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     if (vwo > 1 && !(wo = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwo, L->storage);
     // the even step row-staged: one work-group per row of the contiguous axis, which must
-    // fit one work-group (1024 cells) and its shifted planes' rows 64 KiB of LDS
+    // fit one work-group (1024 cells) and its shifted planes' rows (fluid and fused tracer)
+    // 64 KiB of LDS
     StepFn fr = nullptr;
     const int row_block = (p.len + 63) / 64 * 64;
-    const size_t row_lds = (size_t)bolt_row_slots(L->ndim, L->q) * row_block * store_size(L->storage);
-    if (L->row_even && !can_fuse && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
-        fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, row_block);
+    const size_t row_lds = (size_t)(bolt_row_slots(L->ndim, L->q) + bolt_row_slots(L->ndim, tq)) * row_block *
+                           store_size(L->storage);
+    if (L->row_even && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
+        fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block);
     const long long rows = L->ncell / p.len;
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
     const dim3 wgride((unsigned)wge.nblocks), wgrido((unsigned)wgo.nblocks), wblk(256);
@@ -1374,13 +1376,9 @@ struct BatchParams {
     int write_macro;
 };
 
-template <int COLL, bool EVEN, int TQ>
-__global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
-    const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
-    const Tile t = bp.tiles[b];
-    const InstDesc& d = bp.inst[t.inst];
-    const int iy = (int)t.chunk * (int)blockDim.x + (int)threadIdx.x;
-    if (iy >= d.ny) return;
+// The step parameters of one instance (TQ: its fused tracer's velocity set, or 0).
+template <int TQ>
+__device__ inline StepParams batch_step_params(const InstDesc& d, int write_macro) {
     StepParams p;
     p.n0 = d.nx;
     p.n1 = d.ny;
@@ -1404,10 +1402,34 @@ __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     p.alpha = d.alpha;
     p.c0 = d.c0;
     p.bq_C = d.C;
-    p.write_macro = bp.write_macro;
+    p.write_macro = write_macro;
     p.tC = d.C;
     p.t_om = d.om_g;
+    return p;
+}
+
+template <int COLL, bool EVEN, int TQ>
+__global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
+    const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
+    const Tile t = bp.tiles[b];
+    const InstDesc& d = bp.inst[t.inst];
+    const int iy = (int)t.chunk * (int)blockDim.x + (int)threadIdx.x;
+    if (iy >= d.ny) return;
+    const StepParams p = batch_step_params<TQ>(d, bp.write_macro);
     update_cell<2, 9, COLL, kF32, EVEN, TQ>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
+}
+
+// The even step row-staged (update.hpp row_step): one work-group per (instance, row) tile,
+// launched once per group of tiles whose rows round up to the same multiple of 64 (the
+// group's blockDim.x).  Same per-cell arithmetic as rb_batch_step and the single lattices.
+template <int COLL, int TQ>
+__global__ void __launch_bounds__(1024) rb_batch_row(const BatchParams bp) {
+    extern __shared__ __align__(16) unsigned char rb_row_lds_bytes[];
+    const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
+    const Tile t = bp.tiles[b];
+    const InstDesc& d = bp.inst[t.inst];
+    const StepParams p = batch_step_params<TQ>(d, bp.write_macro);
+    row_step<2, 9, COLL, kF32, TQ>(p, (int)t.row, rb_row_lds_bytes);
 }
 
 struct bolt_batch {
@@ -1415,12 +1437,18 @@ struct bolt_batch {
     int collision = 0;
     std::vector<bolt_lattice*> lat;
     InstDesc* d_inst = nullptr;
-    Tile* d_tiles = nullptr;
+    Tile* d_tiles = nullptr;  // ntiles (instance, row, chunk) tiles, then nrtiles row tiles
     int ntiles = 0;
     int block = 128;     // tile width (cells of the contiguous axis per work-group)
+    int nrtiles = 0;     // (instance, row) tiles of the row-staged even step, grouped by
+                         // row length rounded up to 64: one launch per group
+    struct RowGroup { int first, count, block; };
+    std::vector<RowGroup> rgroups;
+    bool row_even = true;  // even steps row-staged where the rows allow it
     int graph_steps = 100;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0, graph_len = 0;
+    bool graph_row = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t cells = 0;
     bool tracers = true;
@@ -1448,12 +1476,25 @@ static int batch_build_tiles(bolt_batch* B, int block) {
     // before the table does (and waits for the stream).
     batch_drop_graph(B);
     std::vector<Tile> tiles;
+    std::vector<std::vector<Tile>> rt(17);  // row tiles by work-group size / 64 (64 .. 1024)
     for (size_t k = 0; k < B->lat.size(); ++k) {
         if (!B->active.empty() && !B->active[k]) continue;
         const bolt_lattice* L = B->lat[k];
         const int cpr = (L->n[1] + block - 1) / block;
-        for (int r = 0; r < L->n[0]; ++r)
+        const int g = (L->n[1] + 63) / 64;  // rows longer than 1024 cells: no row staging
+        for (int r = 0; r < L->n[0]; ++r) {
             for (int c = 0; c < cpr; ++c) tiles.push_back({(uint16_t)k, (uint16_t)r, (uint16_t)c, 0});
+            if (g <= 16) rt[g].push_back({(uint16_t)k, (uint16_t)r, 0, 0});
+            else rt[0].push_back({(uint16_t)k, (uint16_t)r, 0, 0});
+        }
+    }
+    const size_t nchunk = tiles.size();
+    B->rgroups.clear();
+    const bool too_long = !rt[0].empty();
+    for (int g = 1; g <= 16 && !too_long; ++g) {
+        if (rt[g].empty()) continue;
+        B->rgroups.push_back({(int)(tiles.size() - nchunk), (int)rt[g].size(), 64 * g});
+        tiles.insert(tiles.end(), rt[g].begin(), rt[g].end());
     }
     HIP_TRY(hipStreamSynchronize(B->ctx->stream));  // no launch still reads the old table
     (void)hipFree(B->d_tiles);
@@ -1465,12 +1506,37 @@ static int batch_build_tiles(bolt_batch* B, int block) {
                                B->ctx->stream));
         HIP_TRY(hipStreamSynchronize(B->ctx->stream));
     }
-    B->ntiles = (int)tiles.size();
+    B->ntiles = (int)nchunk;
+    B->nrtiles = (int)(tiles.size() - nchunk);
     B->block = block;
     return BOLT_OK;
 }
 
+// LDS bytes of one row-staged work-group of `block` work-items (fluid and tracer rows
+// shifted along the row; at most 8 rows of 1024 floats: 32 KiB).
+static size_t batch_row_lds(const bolt_batch* B, int block) {
+    return (size_t)(bolt_row_slots(2, 9) + (B->tracers ? bolt_row_slots(2, 5) : 0)) * block * sizeof(float);
+}
+
+// Whether the even step runs row-staged: every active row fits one work-group.
+static bool batch_row(const bolt_batch* B) { return B->row_even && !B->rgroups.empty(); }
+
 static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
+    if (even && batch_row(B)) {
+        for (const auto& g : B->rgroups) {
+            BatchParams bp{B->d_inst, B->d_tiles + B->ntiles + g.first, g.count, write_macro ? 1 : 0};
+            const dim3 grid((unsigned)g.count), blk((unsigned)g.block);
+            const size_t lds = batch_row_lds(B, g.block);
+            if (B->tracers) {
+                if (B->collision == BOLT_BGK) hipLaunchKernelGGL((rb_batch_row<0, 5>), grid, blk, lds, s, bp);
+                else hipLaunchKernelGGL((rb_batch_row<1, 5>), grid, blk, lds, s, bp);
+            } else {
+                if (B->collision == BOLT_BGK) hipLaunchKernelGGL((rb_batch_row<0, 0>), grid, blk, lds, s, bp);
+                else hipLaunchKernelGGL((rb_batch_row<1, 0>), grid, blk, lds, s, bp);
+            }
+        }
+        return;
+    }
     BatchParams bp{B->d_inst, B->d_tiles, B->ntiles, write_macro ? 1 : 0};
     const dim3 grid((unsigned)B->ntiles), blk((unsigned)B->block);
 #define BOLT_BATCH_LAUNCH(C, E, TQ) hipLaunchKernelGGL((rb_batch_step<C, E, TQ>), grid, blk, 0, s, bp)
@@ -1496,6 +1562,7 @@ static int batch_capture(bolt_batch* B, int len) {
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     B->graph_block = B->block;
     B->graph_len = len;
+    B->graph_row = batch_row(B);
     return BOLT_OK;
 }
 
@@ -1507,7 +1574,7 @@ static int batch_enqueue(bolt_batch* B, uint64_t n) {
     const uint64_t len = (uint64_t)B->graph_steps;
     uint64_t done = 0;
     if (len > 0 && n > len + 2) {  // graph_steps 0: every step is a direct launch
-        if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len) {
+        if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len || B->graph_row != batch_row(B)) {
             int rc = batch_capture(B, (int)len);
             if (rc) return rc;
         }
@@ -1689,6 +1756,13 @@ int bolt_batch_cells(const bolt_batch* B, uint64_t* out) {
     return BOLT_OK;
 }
 
+int bolt_batch_set_row_staging(bolt_batch* B, int on) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    B->row_even = on != 0;
+    batch_drop_graph(B);
+    return BOLT_OK;
+}
+
 int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (block < 64 || block > 256 || block % 64 != 0)
@@ -1727,27 +1801,33 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     }
     std::string rep;
     int best_block = B->block, best_len = B->graph_steps;
+    bool best_row = B->row_even;
     double best = 1e300;
     const int blocks[] = {64, 128, 256};
     const int lens[] = {20, 100, 400};
-    for (int blk : blocks) {
-        if (rc) break;
-        for (int gl : lens) {
+    for (int row = 1; row >= 0 && !rc; --row) {
+        rc = bolt_batch_set_row_staging(B, row);
+        for (int blk : blocks) {
             if (rc) break;
-            rc = bolt_batch_set_shape(B, blk, gl);
-            if (rc) break;
-            double ms = 0;
-            rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);  // warm + capture
-            if (!rc) rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);
-            if (rc) break;
-            const double per = ms / std::max(steps, gl + 4);
-            char line[96];
-            snprintf(line, sizeof line, "block %d graph %d: %.4f ms/step\n", blk, gl, per);
-            rep += line;
-            if (per < best) {
-                best = per;
-                best_block = blk;
-                best_len = gl;
+            for (int gl : lens) {
+                if (rc) break;
+                rc = bolt_batch_set_shape(B, blk, gl);
+                if (rc) break;
+                double ms = 0;
+                rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);  // warm + capture
+                if (!rc) rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);
+                if (rc) break;
+                const double per = ms / std::max(steps, gl + 4);
+                char line[96];
+                snprintf(line, sizeof line, "block %d graph %d rows %s: %.4f ms/step\n", blk, gl,
+                         row ? "staged" : "direct", per);
+                rep += line;
+                if (per < best) {
+                    best = per;
+                    best_block = blk;
+                    best_len = gl;
+                    best_row = row != 0;
+                }
             }
         }
     }
@@ -1760,9 +1840,11 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     for (auto& sv : saved) (void)hipFree(sv.src);
     if (rc) return rc;
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "tune: restoring the batch state failed: %s", hipGetErrorString(e));
-    rc = bolt_batch_set_shape(B, best_block, best_len);
+    rc = bolt_batch_set_row_staging(B, best_row ? 1 : 0);
+    if (!rc) rc = bolt_batch_set_shape(B, best_block, best_len);
     char line[96];
-    snprintf(line, sizeof line, "chosen: block %d graph %d\n", best_block, best_len);
+    snprintf(line, sizeof line, "chosen: block %d graph %d rows %s\n", best_block, best_len,
+             best_row ? "staged" : "direct");
     rep += line;
     if (report && len) snprintf(report, len, "%s", rep.c_str());
     return rc;

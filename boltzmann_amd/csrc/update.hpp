@@ -609,9 +609,9 @@ constexpr int waves_target() {
 }
 
 // One non-wall cell of one time step: fluid update through the accessor `a` (its AA
-// locations), optionally followed by one tracer (TQ != 0, addressed through nb).
-template <int D, int Q, int COLL, int S, bool EVEN, int TQ, class A>
-__device__ inline void update_cell_with(const StepParams& p, const A& a, const Nbr<D>& nb, int flag,
+// locations), optionally followed by one tracer (TQ != 0) through the accessor `ga`.
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ, class A, class GA>
+__device__ inline void update_cell_with(const StepParams& p, const A& a, const GA& ga, int flag,
                                         uint32_t cell) {
     using V = VSet<D, Q>;
     using ST = Store<S>;
@@ -647,12 +647,12 @@ __device__ inline void update_cell_with(const StepParams& p, const A& a, const N
         if constexpr (TQ != 0) {
             using W = VSet<D, TQ>;
             double g[TQ];
-            aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+            aa_load_raw<W, EVEN>(ga, g);
             const bool tfixed = (flag & 8) != 0;
             double rt;
             f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
                       tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
-            aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+            aa_store_raw<W, EVEN>(ga, g);
             elem<double>(p.tC, cell) = rt;
         }
     } else {
@@ -727,12 +727,12 @@ __device__ inline void update_cell_with(const StepParams& p, const A& a, const N
             static_assert(S == kF32, "fused tracers use float storage");
             using W = VSet<D, TQ>;
             float g[TQ];
-            aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+            aa_load_raw<W, EVEN>(ga, g);
             const bool tfixed = (flag & 8) != 0;
             float rt;
             ref_ad<W>(g, rt, u, tfixed, p.t_om,
                       tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
-            aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+            aa_store_raw<W, EVEN>(ga, g);
             elem<float>(p.tC, cell) = rt;
         }
     }
@@ -744,7 +744,8 @@ __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, 
     const int flag = elem<uint8_t>((const void*)p.flags, cell);
     if (flag & 1) return;
     const Nbr<D> nb(ix, iy, iz, p);
-    update_cell_with<D, Q, COLL, S, EVEN, TQ>(p, CellAA<D, EVEN>{p.fpl, nb, cell}, nb, flag, cell);
+    update_cell_with<D, Q, COLL, S, EVEN, TQ>(p, CellAA<D, EVEN>{p.fpl, nb, cell},
+                                              CellAA<D, EVEN>{p.gpl, nb, cell}, flag, cell);
 }
 
 template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
@@ -767,8 +768,9 @@ __global__ void __launch_bounds__(256)
 // LDS, where work-item z reads and later writes element (z - c_k,row) mod len -- a
 // bijection, so each element has exactly one owner and needs no barrier between its read
 // and its write.  After a barrier the shifted planes' rows are stored from LDS at element
-// z.  Elements owned by wall cells, which neither read nor write, go back as loaded.  The
-// per-cell arithmetic is update_cell_with, shared with lbm_step, so both kernels give
+// z.  Elements owned by wall cells, which neither read nor write, go back as loaded.  A
+// fused tracer's planes are staged the same way.  The per-cell arithmetic is
+// update_cell_with, shared with lbm_step (and the batch kernels), so the kernels give
 // identical bits.
 
 // LDS row of plane k (planes with no component along the contiguous axis have none).
@@ -811,21 +813,22 @@ struct RowAA {
     }
 };
 
-// BMAX: the largest work-group the instantiation serves (256 or 1024).  Launched with
-// p.nblocks = rows, blockDim.x = len rounded up to 64 and row_slots * blockDim.x elements
-// of dynamic LDS.
-template <int D, int Q, int COLL, int S, int BMAX>
-__global__ void __launch_bounds__(BMAX)
-    __attribute__((amdgpu_waves_per_eu(BMAX > 256 ? BMAX / 256 : waves_target<Q, S, 0>())))
-    lbm_step_row(const StepParams p) {
+// One row of the row-staged even step, fluid (Q) and, with TQ != 0, the fused tracer: the
+// work-group owns row `row`, work-item z = threadIdx.x its cell z (z < p.len), and `lds`
+// holds (row_slots<fluid> + row_slots<tracer>) * blockDim.x elements of the storage type.
+// Every work-item of the work-group must call it (it holds two barriers).
+template <int D, int Q, int COLL, int S, int TQ>
+__device__ inline void row_step(const StepParams& p, int row, unsigned char* lds_bytes) {
     using V = VSet<D, Q>;
+    using W = VSet<D, TQ != 0 ? TQ : (D == 2 ? 5 : 7)>;  // the tracer's set (unused if TQ == 0)
     using T = typename Store<S>::T;
-    extern __shared__ __align__(16) unsigned char row_lds_bytes[];
-    T* lds = (T*)row_lds_bytes;
-    const int row = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    static_assert(TQ == 0 || S == kF32 || S == kF64, "fused tracers use float or double storage");
+    constexpr int GQ = TQ != 0 ? TQ : 1;
     const int z = (int)threadIdx.x;
     const int stride = (int)blockDim.x;
     const bool live = z < p.len;
+    T* lds = (T*)lds_bytes;
+    T* glds = lds + row_slots<V>() * stride;
     int ix, iy;
     if (D == 2) {
         ix = row;
@@ -837,7 +840,7 @@ __global__ void __launch_bounds__(BMAX)
     const Nbr<D> nb(ix, iy, z, p);
     const uint32_t cell = nb.row_at(0, 0) + (uint32_t)z;
     int flag = 1;
-    T own[Q];
+    T own[Q], gown[GQ];
     if (live) {
         flag = elem<uint8_t>((const void*)p.flags, cell);
 #pragma unroll
@@ -846,11 +849,20 @@ __global__ void __launch_bounds__(BMAX)
             if (V::c(k, D - 1) == 0) own[k] = v;
             else lds[row_slot<V>(k) * stride + z] = v;
         }
+        if constexpr (TQ != 0) {
+#pragma unroll
+            for (int k = 0; k < TQ; ++k) {
+                const T v = pop_load<T>(p.gpl[k], nb.row_at(-W::c(k, 0), D == 3 ? -W::c(k, 1) : 0) + (uint32_t)z);
+                if (W::c(k, D - 1) == 0) gown[k] = v;
+                else glds[row_slot<W>(k) * stride + z] = v;
+            }
+        }
     }
     __syncthreads();
     if (live && !(flag & 1))
-        update_cell_with<D, Q, COLL, S, true, 0>(p, RowAA<D, T>{p.fpl, nb, own, lds, stride, z, p.len}, nb,
-                                                 flag, cell);
+        update_cell_with<D, Q, COLL, S, true, TQ>(p, RowAA<D, T>{p.fpl, nb, own, lds, stride, z, p.len},
+                                                  RowAA<D, T>{p.gpl, nb, gown, glds, stride, z, p.len}, flag,
+                                                  cell);
     __syncthreads();
     if (live) {
 #pragma unroll
@@ -858,7 +870,25 @@ __global__ void __launch_bounds__(BMAX)
             if (V::c(k, D - 1) != 0)
                 pop_store<T>(p.fpl[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z,
                              lds[row_slot<V>(k) * stride + z]);
+        if constexpr (TQ != 0) {
+#pragma unroll
+            for (int k = 0; k < TQ; ++k)
+                if (W::c(k, D - 1) != 0)
+                    pop_store<T>(p.gpl[k], nb.row_at(-W::c(k, 0), D == 3 ? -W::c(k, 1) : 0) + (uint32_t)z,
+                                 glds[row_slot<W>(k) * stride + z]);
+        }
     }
+}
+
+// BMAX: the largest work-group the instantiation serves (256 or 1024).  Launched with
+// p.nblocks = rows, blockDim.x = len rounded up to 64 and (row_slots of the fluid + row_slots
+// of the fused tracer) * blockDim.x elements of dynamic LDS.
+template <int D, int Q, int COLL, int S, int BMAX, int TQ>
+__global__ void __launch_bounds__(BMAX)
+    __attribute__((amdgpu_waves_per_eu(BMAX > 256 ? BMAX / 256 : waves_target<Q, S, TQ>())))
+    lbm_step_row(const StepParams p) {
+    extern __shared__ __align__(16) unsigned char row_lds_bytes[];
+    row_step<D, Q, COLL, S, TQ>(p, xcd_swizzle((int)blockIdx.x, p.nblocks), row_lds_bytes);
 }
 
 // Advection-diffusion tracer on its own, reading the fluid velocity written this step.

@@ -240,12 +240,18 @@ int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double*
 int bolt_batch_residuals(bolt_batch* b, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
+/* on (the default): even steps row-staged, one work-group per (instance, row), as
+ * bolt_lattice_set_row_staging, when every active row has at most 1024 cells (one launch
+ * per group of rows of the same length rounded up to 64); off: every step through the
+ * tiles of set_shape.  Results do not depend on the choice. */
+int bolt_batch_set_row_staging(bolt_batch* b, int on);
 /* instances that later steps advance (active[n], non-zero: advance); an inactive instance
  * keeps its state and can still be read, reduced or checked for its residual.  Change it
  * only between step calls (each of an even count, so every instance stays in its natural
  * population layout). */
 int bolt_batch_set_active(bolt_batch* b, const uint8_t* active, int n);
-/* autotuner: times the candidate shapes on this batch (state restored afterwards), keeps
+/* autotuner: times the candidate shapes on this batch (state restored afterwards) -- work-
+ * group width 64/128/256, 20/100/400 steps per graph, even steps row-staged or not --, keeps
  * the fastest and writes a per-candidate report into report[len]. */
 int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
 

@@ -116,3 +116,99 @@ def test_row_staged_with_every_width(storage):
         staged.iterate(4)
         direct.iterate(4)
         _same(staged, direct)
+
+
+@pytest.mark.parametrize("counts", [(6, 256), (5, 200), (3, 1024), (4, 37), (2, 3, 68), (3, 2, 8)])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_row_staged_fused_tracer(counts, storage):
+    # float and double storage fuse the first tracer into the fluid step; its planes are
+    # staged with the fluid's, with fixed-scalar cells and a Boussinesq coupling
+    import boltzmann_amd
+    from boltzmann_amd import core
+    d = len(counts)
+    rng = np.random.default_rng(19)
+    flags = np.zeros(counts, np.int32)
+    flags[rng.random(counts) < 0.1] = 1
+    flags[(rng.random(counts) < 0.05) & (flags == 0)] = 6
+    flags[(rng.random(counts) < 0.08) & (flags != 1)] |= 8
+    vel = (0.03 * rng.standard_normal(counts + (d,))).astype(np.float32)
+    C = rng.random(counts).astype(np.float32)
+    sims = []
+    for staged in (True, False):
+        s = boltzmann_amd.Simulation("gpu", list(counts), omega_obj(OMEGAS[1]), storage=storage)
+        s.set_row_staging(staged)
+        s.cells.flags[...] = flags
+        s.fluid.vel[...] = vel
+        t = s.add_tracer("T", core.bgk(0.7))
+        t.val[...] = C
+        s.add_boussinesq_coupling(t, 0.02, 0.5, np.array([0.0, -1e-3, 0.0][:d], np.float32))
+        sims.append(s)
+    a, b = sims
+    for n in (2, 4):
+        a.iterate(n)
+        b.iterate(n)
+        _same(a, b)
+        np.testing.assert_array_equal(a.read_field("g", "T", dtype="f64"), b.read_field("g", "T", dtype="f64"))
+        np.testing.assert_array_equal(a.read_field("C", "T", dtype="f64"), b.read_field("C", "T", dtype="f64"))
+
+
+@pytest.mark.parametrize("counts", [(6, 200), (3, 4, 100)])
+def test_row_staged_fused_tracer_f32_bit_exact_against_oracle(counts):
+    q = 9 if len(counts) == 2 else 27
+    d = len(counts)
+    sim, ref, flags = make_pair(counts, OMEGAS[1], q, seed=23, tracers=(("T", f32(1 / 0.7)),),
+                                couplings=(("T", f32(0.02), f32(0.5), [0.0, -1e-3, 0.0][:d]),), scalar=0.06)
+    sim.set_row_staging(True)
+    nw = (flags & 1) == 0
+    for _ in range(3):
+        sim.iterate(4)
+        ref.iterate(4)
+        np.testing.assert_array_equal(sim.read_field("f"), ref.f)
+        np.testing.assert_array_equal(sim.read_field("g", "T"), ref.tracers["T"].g)
+        np.testing.assert_array_equal(sim.get_tracer("T").val[nw], ref.tracers["T"].C[nw])
+
+
+@pytest.mark.parametrize("coll", ["bgk", "trt"])
+@pytest.mark.parametrize("shape", [(64, 0), (128, 400)])
+def test_batch_row_staged_equals_direct(coll, shape):
+    # uneven instances (rows of 17 to 130 cells in one work-group size), direct launches and
+    # replayed graphs, and an instance switched off and on again
+    from test_gpu_batch import specs, build_batch
+    sp = specs(coll)
+    a, b = build_batch(sp, coll), build_batch(sp, coll)
+    a.set_row_staging(True)
+    b.set_row_staging(False)
+    for x in (a, b):
+        x.set_shape(*shape)
+        x.step(24)
+        active = np.ones(len(sp), bool)
+        active[2] = False
+        x.set_active(active)
+        x.step(10)
+        x.set_active(np.ones(len(sp), bool))
+        x.step(444)
+    for k in range(len(sp)):
+        for field in ("f", "g", "C", "vel", "rho"):
+            np.testing.assert_array_equal(a.read(k, field), b.read(k, field))
+
+
+def test_batch_with_a_row_too_long_for_staging():
+    # one instance's rows exceed 1024 cells: the batch's even steps stay direct; the other
+    # shapes exercise one launch per row-length group when they are on their own
+    import test_gpu_batch as tb
+    shapes = tb.SHAPES
+    try:
+        for sh in ([(4, 1030), (5, 60)], [(3, 64), (4, 65), (2, 1024), (3, 200)]):
+            tb.SHAPES = sh
+            sp = tb.specs("trt")
+            a, b = tb.build_batch(sp, "trt"), tb.build_batch(sp, "trt")
+            a.set_row_staging(True)
+            b.set_row_staging(False)
+            for x in (a, b):
+                x.step(30)
+            for k, s in enumerate(sp):
+                np.testing.assert_array_equal(a.read(k, "f"), b.read(k, "f"))
+                np.testing.assert_array_equal(a.read(k, "g"), b.read(k, "g"))
+                np.testing.assert_array_equal(a.read(k, "f"), tb.oracle_run(s, "trt", 30).f)
+    finally:
+        tb.SHAPES = shapes
