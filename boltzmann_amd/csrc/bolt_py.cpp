@@ -132,6 +132,15 @@ struct Core {
     }
 };
 
+// True / False / "auto" -> BOLT_ROW_ON / BOLT_ROW_OFF / BOLT_ROW_AUTO
+int row_mode(const py::object& mode) {
+    if (py::isinstance<py::str>(mode)) {
+        if (mode.cast<std::string>() == "auto") return BOLT_ROW_AUTO;
+        throw BoltError("row staging: True, False or 'auto'");
+    }
+    return mode.cast<bool>() ? BOLT_ROW_ON : BOLT_ROW_OFF;
+}
+
 int device_kind(const std::string& dev) {
     std::string u;
     for (char c : dev) u.push_back((char)toupper(c));
@@ -635,11 +644,11 @@ PYBIND11_MODULE(_boltzmann, m) {
             // cells per lane of the fluid step (0: chosen per lattice); results do not change
             check(bolt_lattice_set_vector_width(s.core->lat, vw), "set_vector_width");
         }, py::arg("vw"))
-        .def("set_row_staging", [](PySimulation& s, bool on) {
-            // even steps through the row-staged kernel where the lattice allows it (default
-            // on); results do not change
-            check(bolt_lattice_set_row_staging(s.core->lat, on ? 1 : 0), "set_row_staging");
-        }, py::arg("on"))
+        .def("set_row_staging", [](PySimulation& s, py::object mode) {
+            // the even step's kernel: True (row-staged wherever it can run), False (direct)
+            // or "auto" (the default: where it measured faster); results do not change
+            check(bolt_lattice_set_row_staging(s.core->lat, row_mode(mode)), "set_row_staging");
+        }, py::arg("mode"))
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;
             check(bolt_lattice_device_bytes(s.core->lat, &b), "device_bytes");
@@ -752,9 +761,9 @@ PYBIND11_MODULE(_boltzmann, m) {
         .def("set_shape", [](PyBatch& b, int block, int graph_steps) {
             check(bolt_batch_set_shape(b.core->batch, block, graph_steps), "set_shape");
         }, py::arg("block"), py::arg("graph_steps"))
-        .def("set_row_staging", [](PyBatch& b, bool on) {
-            check(bolt_batch_set_row_staging(b.core->batch, on ? 1 : 0), "set_row_staging");
-        }, py::arg("on"))
+        .def("set_row_staging", [](PyBatch& b, py::object mode) {
+            check(bolt_batch_set_row_staging(b.core->batch, row_mode(mode)), "set_row_staging");
+        }, py::arg("mode"))
         .def("set_active", [](PyBatch& b, py::array_t<bool, py::array::c_style | py::array::forcecast> a) {
             std::vector<uint8_t> v(a.data(), a.data() + a.size());
             check(bolt_batch_set_active(b.core->batch, v.data(), (int)v.size()), "set_active");

@@ -80,7 +80,7 @@ struct bolt_lattice {
     std::vector<Coupling> couplings;
     int block = 0;
     int vw = 0;                  // cells per lane of the fluid step (0: chosen per lattice)
-    bool row_even = true;        // even steps row-staged (lbm_step_row) where they qualify
+    int row_even = BOLT_ROW_AUTO;  // even steps row-staged (lbm_step_row): BOLT_ROW_*
     double* partials = nullptr;  // reduction scratch
     double* result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -933,7 +933,15 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const int row_block = (p.len + 63) / 64 * 64;
     const size_t row_lds = (size_t)(bolt_row_slots(L->ndim, L->q) + bolt_row_slots(L->ndim, tq)) * row_block *
                            store_size(L->storage);
-    if (L->row_even && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
+    // BOLT_ROW_AUTO takes it where it measured faster on MI355X (one session, D2Q9 and D3Q27
+    // TRT): rows of a multiple of 64 cells, whose rows start on a cache line, so that no line
+    // is shared between work-groups (+0.2 to +10.6 %; rows of 1000 cells lost 2 %), except
+    // D3Q27 rows over 512 cells in the 2-byte formats, whose 1024-work-item groups fit one per
+    // CU (-11 %)
+    const bool two_byte = L->storage == kF16S || L->storage == kBF16S;
+    const bool row_pays = p.len % 64 == 0 && !(L->ndim == 3 && two_byte && row_block > 512);
+    const bool row_wanted = L->row_even == BOLT_ROW_ON || (L->row_even == BOLT_ROW_AUTO && row_pays);
+    if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
         fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block);
     const long long rows = L->ncell / p.len;
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
@@ -1089,9 +1097,11 @@ int bolt_lattice_set_vector_width(bolt_lattice* L, int vw) {
     return BOLT_OK;
 }
 
-int bolt_lattice_set_row_staging(bolt_lattice* L, int on) {
+int bolt_lattice_set_row_staging(bolt_lattice* L, int mode) {
     if (!L) return fail(BOLT_EINVAL, "null lattice");
-    L->row_even = on != 0;
+    if (mode != BOLT_ROW_OFF && mode != BOLT_ROW_ON && mode != BOLT_ROW_AUTO)
+        return fail(BOLT_EINVAL, "row staging mode must be BOLT_ROW_OFF, _ON or _AUTO");
+    L->row_even = mode;
     return BOLT_OK;
 }
 
@@ -1444,7 +1454,7 @@ struct bolt_batch {
                          // row length rounded up to 64: one launch per group
     struct RowGroup { int first, count, block; };
     std::vector<RowGroup> rgroups;
-    bool row_even = true;  // even steps row-staged where the rows allow it
+    bool row_even = false; // even steps row-staged (the autotuner tries both)
     int graph_steps = 100;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0, graph_len = 0;
@@ -1756,9 +1766,13 @@ int bolt_batch_cells(const bolt_batch* B, uint64_t* out) {
     return BOLT_OK;
 }
 
-int bolt_batch_set_row_staging(bolt_batch* B, int on) {
+int bolt_batch_set_row_staging(bolt_batch* B, int mode) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
-    B->row_even = on != 0;
+    if (mode != BOLT_ROW_OFF && mode != BOLT_ROW_ON && mode != BOLT_ROW_AUTO)
+        return fail(BOLT_EINVAL, "row staging mode must be BOLT_ROW_OFF, _ON or _AUTO");
+    // batches measured slower row-staged on MI355X (warm-start channels and the RB sweep):
+    // BOLT_ROW_AUTO keeps them direct unless the autotuner finds otherwise
+    B->row_even = mode == BOLT_ROW_ON;
     batch_drop_graph(B);
     return BOLT_OK;
 }
@@ -1806,7 +1820,7 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     const int blocks[] = {64, 128, 256};
     const int lens[] = {20, 100, 400};
     for (int row = 1; row >= 0 && !rc; --row) {
-        rc = bolt_batch_set_row_staging(B, row);
+        rc = bolt_batch_set_row_staging(B, row ? BOLT_ROW_ON : BOLT_ROW_OFF);
         for (int blk : blocks) {
             if (rc) break;
             for (int gl : lens) {
@@ -1840,7 +1854,7 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     for (auto& sv : saved) (void)hipFree(sv.src);
     if (rc) return rc;
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "tune: restoring the batch state failed: %s", hipGetErrorString(e));
-    rc = bolt_batch_set_row_staging(B, best_row ? 1 : 0);
+    rc = bolt_batch_set_row_staging(B, best_row ? BOLT_ROW_ON : BOLT_ROW_OFF);
     if (!rc) rc = bolt_batch_set_shape(B, best_block, best_len);
     char line[96];
     snprintf(line, sizeof line, "chosen: block %d graph %d rows %s\n", best_block, best_len,

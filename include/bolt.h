@@ -201,14 +201,20 @@ int bolt_lattice_set_block(bolt_lattice* lat, int block);
  * lane.  Results do not depend on the choice (bit-identical). */
 int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
 /* Launch tuning (no counterpart in the reference, whose kernels take one cell per
- * work-item): on (the default) runs the even (streaming) step row-staged where the lattice
- * allows it -- one work-group per row of the contiguous axis, every population access an
- * aligned row, the shift along the row taken through LDS.  It applies when one cell per
- * lane is chosen for the even step, no tracer is fused into the fluid step, the row has at
- * most 1024 cells and its shifted planes' rows fit 64 KiB of LDS; otherwise, and when off,
- * the even step addresses each neighbour's slot directly.  Results do not depend on the
- * choice (bit-identical). */
-int bolt_lattice_set_row_staging(bolt_lattice* lat, int on);
+ * work-item): whether the even (streaming) step runs row-staged -- one work-group per row of
+ * the contiguous axis, every population access an aligned row, the shift along the row
+ * taken through LDS -- or addresses each neighbour's slot directly.  The row-staged step can
+ * run when one cell per lane is chosen for the even step, the row has at most 1024 cells
+ * and its shifted planes' rows (fluid and fused tracer) fit 64 KiB of LDS.
+ *   BOLT_ROW_ON    wherever it can run
+ *   BOLT_ROW_OFF   never
+ *   BOLT_ROW_AUTO  (the default) where it measured faster: rows of a multiple of 64 cells,
+ *                  except D3Q27 rows over 512 cells in the 2-byte formats
+ * Results do not depend on the choice (bit-identical). */
+#define BOLT_ROW_OFF 0
+#define BOLT_ROW_ON 1
+#define BOLT_ROW_AUTO 2
+int bolt_lattice_set_row_staging(bolt_lattice* lat, int mode);
 
 /* ---- batches: many small 2D Rayleigh-Benard-type lattices in one launch per step -----
  * No counterpart in the reference (it runs one Simulation per process).  Instance k is a
@@ -240,11 +246,12 @@ int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double*
 int bolt_batch_residuals(bolt_batch* b, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
-/* on (the default): even steps row-staged, one work-group per (instance, row), as
+/* BOLT_ROW_ON: even steps row-staged, one work-group per (instance, row), as
  * bolt_lattice_set_row_staging, when every active row has at most 1024 cells (one launch
- * per group of rows of the same length rounded up to 64); off: every step through the
- * tiles of set_shape.  Results do not depend on the choice. */
-int bolt_batch_set_row_staging(bolt_batch* b, int on);
+ * per group of rows of the same length rounded up to 64).  BOLT_ROW_OFF and BOLT_ROW_AUTO
+ * (the default; batches measured slower row-staged): every step through the tiles of
+ * set_shape.  bolt_batch_tune tries both.  Results do not depend on the choice. */
+int bolt_batch_set_row_staging(bolt_batch* b, int mode);
 /* instances that later steps advance (active[n], non-zero: advance); an inactive instance
  * keeps its state and can still be read, reduced or checked for its residual.  Change it
  * only between step calls (each of an even count, so every instance stays in its natural

@@ -212,3 +212,25 @@ def test_batch_with_a_row_too_long_for_staging():
                 np.testing.assert_array_equal(a.read(k, "f"), tb.oracle_run(s, "trt", 30).f)
     finally:
         tb.SHAPES = shapes
+
+
+@pytest.mark.parametrize("counts", [(5, 256), (5, 200), (2, 3, 1024), (2, 2, 64)])
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_row_staging_modes_agree(counts, storage):
+    # "auto" (the default: row-staged where it measured faster), on and off give the same bits
+    import boltzmann_amd
+    from boltzmann_amd import core
+    d = len(counts)
+    vel = (0.03 * np.random.default_rng(29).standard_normal(counts + (d,))).astype(np.float32)
+    sims = []
+    for mode in ("auto", True, False):
+        s = boltzmann_amd.Simulation("gpu", list(counts), core.trt(0.6), storage=storage)
+        s.set_row_staging(mode)
+        s.fluid.vel[...] = vel
+        sims.append(s)
+    for s in sims:
+        s.iterate(6)
+    _same(sims[0], sims[1])
+    _same(sims[0], sims[2])
+    with pytest.raises(boltzmann_amd.BoltError):
+        sims[0].set_row_staging("sometimes")
