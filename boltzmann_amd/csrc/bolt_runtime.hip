@@ -933,13 +933,13 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const int row_block = (p.len + 63) / 64 * 64;
     const size_t row_lds = (size_t)(bolt_row_slots(L->ndim, L->q) + bolt_row_slots(L->ndim, tq)) * row_block *
                            store_size(L->storage);
-    // BOLT_ROW_AUTO takes it where it measured faster on MI355X (one session, D2Q9 and D3Q27
-    // TRT): rows of a multiple of 64 cells, whose rows start on a cache line, so that no line
-    // is shared between work-groups (+0.2 to +10.6 %; rows of 1000 cells lost 2 %), except
-    // D3Q27 rows over 512 cells in the 2-byte formats, whose 1024-work-item groups fit one per
-    // CU (-11 %)
+    // BOLT_ROW_AUTO takes it where it measured faster on MI355X (D2Q9 and D3Q27 TRT): rows
+    // that fill whole 128-byte cache lines, so that no line is shared between work-groups
+    // (+0.2 to +10.6 %; rows of 1000 float cells lost 2 %), except D3Q27 rows over 512 cells
+    // in the 2-byte formats, whose 1024-work-item groups fit one per CU (-11 %)
     const bool two_byte = L->storage == kF16S || L->storage == kBF16S;
-    const bool row_pays = p.len % 64 == 0 && !(L->ndim == 3 && two_byte && row_block > 512);
+    const bool row_pays = (size_t)p.len * store_size(L->storage) % 128 == 0 &&
+                          !(L->ndim == 3 && two_byte && row_block > 512);
     const bool row_wanted = L->row_even == BOLT_ROW_ON || (L->row_even == BOLT_ROW_AUTO && row_pays);
     if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
         fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block);
@@ -1386,6 +1386,17 @@ struct BatchParams {
     int write_macro;
 };
 
+// A pointer read from a descriptor in device memory, as a pointer into global memory: the
+// compiler then addresses it with the global (scalar-base) memory instructions instead of
+// the flat ones it must use for a pointer of unknown address space -- which also count
+// against the LDS wait counter and would order every LDS access behind them.
+template <class T>
+__device__ inline T* global_ptr(T* p) {
+    // through an integer: a cast of the generic pointer to the global space and back would
+    // fold away
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 // The step parameters of one instance (TQ: its fused tracer's velocity set, or 0).
 template <int TQ>
 __device__ inline StepParams batch_step_params(const InstDesc& d, int write_macro) {
@@ -1394,16 +1405,18 @@ __device__ inline StepParams batch_step_params(const InstDesc& d, int write_macr
     p.n1 = d.ny;
     p.n2 = 1;
     p.len = d.ny;
+    char* const f = global_ptr(d.f);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) p.fpl[i] = d.f + (size_t)i * d.plane_bytes;
+    for (int i = 0; i < 9; ++i) p.fpl[i] = f + (size_t)i * d.plane_bytes;
     if constexpr (TQ != 0) {
+        char* const g = global_ptr(d.g);
 #pragma unroll
-        for (int i = 0; i < TQ; ++i) p.gpl[i] = d.g + (size_t)i * d.plane_bytes;
+        for (int i = 0; i < TQ; ++i) p.gpl[i] = g + (size_t)i * d.plane_bytes;
     }
-    p.rho = d.rho;
-    p.vel[0] = d.vel0;
-    p.vel[1] = d.vel1;
-    p.flags = d.flags;
+    p.rho = global_ptr(d.rho);
+    p.vel[0] = global_ptr(d.vel0);
+    p.vel[1] = global_ptr(d.vel1);
+    p.flags = global_ptr(d.flags);
     p.om_p = d.om_p;
     p.om_n = d.om_n;
     p.acc_mode = d.acc_mode;
@@ -1411,9 +1424,9 @@ __device__ inline StepParams batch_step_params(const InstDesc& d, int write_macr
     p.g[1] = d.gy;
     p.alpha = d.alpha;
     p.c0 = d.c0;
-    p.bq_C = d.C;
+    p.bq_C = global_ptr(d.C);
     p.write_macro = write_macro;
-    p.tC = d.C;
+    p.tC = global_ptr(d.C);
     p.t_om = d.om_g;
     return p;
 }
@@ -1454,7 +1467,8 @@ struct bolt_batch {
                          // row length rounded up to 64: one launch per group
     struct RowGroup { int first, count, block; };
     std::vector<RowGroup> rgroups;
-    bool row_even = false; // even steps row-staged (the autotuner tries both)
+    int row_mode = BOLT_ROW_AUTO;  // even steps row-staged: BOLT_ROW_* (the autotuner tries both)
+    bool rows_whole_lines = false; // every active row fills whole 128-byte lines
     int graph_steps = 100;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0, graph_len = 0;
@@ -1499,6 +1513,10 @@ static int batch_build_tiles(bolt_batch* B, int block) {
         }
     }
     const size_t nchunk = tiles.size();
+    B->rows_whole_lines = true;
+    for (size_t k = 0; k < B->lat.size(); ++k)
+        if ((B->active.empty() || B->active[k]) && (size_t)B->lat[k]->n[1] * sizeof(float) % 128 != 0)
+            B->rows_whole_lines = false;
     B->rgroups.clear();
     const bool too_long = !rt[0].empty();
     for (int g = 1; g <= 16 && !too_long; ++g) {
@@ -1528,8 +1546,13 @@ static size_t batch_row_lds(const bolt_batch* B, int block) {
     return (size_t)(bolt_row_slots(2, 9) + (B->tracers ? bolt_row_slots(2, 5) : 0)) * block * sizeof(float);
 }
 
-// Whether the even step runs row-staged: every active row fits one work-group.
-static bool batch_row(const bolt_batch* B) { return B->row_even && !B->rgroups.empty(); }
+// Whether the even step runs row-staged: every active row fits one work-group, and the mode
+// asks for it (BOLT_ROW_AUTO: where the rows fill whole cache lines, as for single lattices;
+// measured faster on the RB sweep and the warm-start channels).
+static bool batch_row(const bolt_batch* B) {
+    if (B->rgroups.empty()) return false;
+    return B->row_mode == BOLT_ROW_ON || (B->row_mode == BOLT_ROW_AUTO && B->rows_whole_lines);
+}
 
 static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
     if (even && batch_row(B)) {
@@ -1770,9 +1793,7 @@ int bolt_batch_set_row_staging(bolt_batch* B, int mode) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (mode != BOLT_ROW_OFF && mode != BOLT_ROW_ON && mode != BOLT_ROW_AUTO)
         return fail(BOLT_EINVAL, "row staging mode must be BOLT_ROW_OFF, _ON or _AUTO");
-    // batches measured slower row-staged on MI355X (warm-start channels and the RB sweep):
-    // BOLT_ROW_AUTO keeps them direct unless the autotuner finds otherwise
-    B->row_even = mode == BOLT_ROW_ON;
+    B->row_mode = mode;
     batch_drop_graph(B);
     return BOLT_OK;
 }
@@ -1815,7 +1836,7 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     }
     std::string rep;
     int best_block = B->block, best_len = B->graph_steps;
-    bool best_row = B->row_even;
+    bool best_row = batch_row(B);
     double best = 1e300;
     const int blocks[] = {64, 128, 256};
     const int lens[] = {20, 100, 400};
@@ -1885,19 +1906,22 @@ constexpr int kResSlices = 16;
 
 __global__ void k_batch_residual(const ResDesc* desc, double* part) {
     const ResDesc r = desc[blockIdx.y];
+    const float* const vel = global_ptr(r.vel);
+    double* const snap = global_ptr(r.snap);
+    const uint8_t* const flags = global_ptr(r.flags);
     double dmax = 0.0, umax = 0.0;
     if (r.active) {
         for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < r.ncell;
              c += (long long)gridDim.x * blockDim.x) {
             double du2 = 0.0, u2 = 0.0;
             for (int a = 0; a < 2; ++a) {
-                const double u = (double)r.vel[(long long)a * r.ncell + c];
-                const double w = u - r.snap[(long long)a * r.ncell + c];
+                const double u = (double)vel[(long long)a * r.ncell + c];
+                const double w = u - snap[(long long)a * r.ncell + c];
                 du2 += w * w;
                 u2 += u * u;
-                r.snap[(long long)a * r.ncell + c] = u;
+                snap[(long long)a * r.ncell + c] = u;
             }
-            if (!(r.flags[c] & 1)) {
+            if (!(flags[c] & 1)) {
                 dmax = fmax(dmax, du2);
                 umax = fmax(umax, u2);
             }

@@ -208,8 +208,8 @@ int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
  * and its shifted planes' rows (fluid and fused tracer) fit 64 KiB of LDS.
  *   BOLT_ROW_ON    wherever it can run
  *   BOLT_ROW_OFF   never
- *   BOLT_ROW_AUTO  (the default) where it measured faster: rows of a multiple of 64 cells,
- *                  except D3Q27 rows over 512 cells in the 2-byte formats
+ *   BOLT_ROW_AUTO  (the default) where it measured faster: rows that fill whole 128-byte
+ *                  cache lines, except D3Q27 rows over 512 cells in the 2-byte formats
  * Results do not depend on the choice (bit-identical). */
 #define BOLT_ROW_OFF 0
 #define BOLT_ROW_ON 1
@@ -246,11 +246,12 @@ int bolt_batch_residual(bolt_batch* b, int inst, uint64_t steps_between, double*
 int bolt_batch_residuals(bolt_batch* b, uint64_t steps_between, double* out);
 /* launch shape: work-group width along the contiguous axis and steps per captured graph */
 int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
-/* BOLT_ROW_ON: even steps row-staged, one work-group per (instance, row), as
- * bolt_lattice_set_row_staging, when every active row has at most 1024 cells (one launch
- * per group of rows of the same length rounded up to 64).  BOLT_ROW_OFF and BOLT_ROW_AUTO
- * (the default; batches measured slower row-staged): every step through the tiles of
- * set_shape.  bolt_batch_tune tries both.  Results do not depend on the choice. */
+/* Even steps row-staged, one work-group per (instance, row), as
+ * bolt_lattice_set_row_staging: BOLT_ROW_ON whenever every active row has at most 1024 cells
+ * (one launch per group of rows of the same length rounded up to 64), BOLT_ROW_AUTO (the
+ * default) when in addition every active row fills whole 128-byte cache lines, BOLT_ROW_OFF
+ * never (every step through the tiles of set_shape).  bolt_batch_tune tries on and off.
+ * Results do not depend on the choice. */
 int bolt_batch_set_row_staging(bolt_batch* b, int mode);
 /* instances that later steps advance (active[n], non-zero: advance); an inactive instance
  * keeps its state and can still be read, reduced or checked for its residual.  Change it
