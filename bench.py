@@ -404,7 +404,7 @@ def memory_ceilings():
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
-def nn_throughput(nx: int = 512, ny: int = 256, batch: int = 8, width: int = 32, depth: int = 6,
+def nn_throughput(nx: int = 512, ny: int = 256, batch: int = 8, width: int = 16, depth: int = 6,
                   samples: int = 64, epochs: int = 2):
     """Training and inference throughput of the warm-start network on random data of the
     512 x 256 channel size (the GEMM kernels' workload), after one warm-up epoch."""
@@ -514,6 +514,9 @@ def main() -> int:
     ap.add_argument("--ws-epochs", type=int, default=20)
     ap.add_argument("--nn-throughput", action="store_true",
                     help="add the warm-start network's training / inference throughput")
+    ap.add_argument("--nn-width", type=int, default=16,
+                    help="channels of the first level of the --nn-throughput network (16: the warm "
+                         "start's default)")
     ap.add_argument("--no-warmstart-bounded", action="store_true",
                     help="skip the bounded 512x256 warm-start leg of the default run (~3 min)")
     ap.add_argument("--vw-sweep", action="store_true",
@@ -587,7 +590,7 @@ def main() -> int:
                       file=sys.stderr, flush=True)
         variants["vector_width_sweep_mlups"] = sweep
     if args.nn_throughput:
-        variants["nn_unet_512x256"] = nn_throughput()
+        variants["nn_unet_512x256"] = nn_throughput(width=args.nn_width)
         print(f"[bench] network: {variants['nn_unet_512x256']}", file=sys.stderr, flush=True)
     if args.warmstart:
         variants["warmstart_channel_512x256_full"] = warmstart_channel(
