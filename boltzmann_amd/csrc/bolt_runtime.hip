@@ -80,7 +80,7 @@ struct bolt_lattice {
     std::vector<Coupling> couplings;
     int block = 0;
     int vw = 0;                  // cells per lane of the fluid step (0: chosen per lattice)
-    int row_even = BOLT_ROW_AUTO;  // even steps row-staged (lbm_step_row): BOLT_ROW_*
+    int row_mode = BOLT_ROW_AUTO;  // even steps row-staged (lbm_step_row): BOLT_ROW_*
     double* partials = nullptr;  // reduction scratch
     double* result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -940,7 +940,7 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     const bool two_byte = L->storage == kF16S || L->storage == kBF16S;
     const bool row_pays = (size_t)p.len * store_size(L->storage) % 128 == 0 &&
                           !(L->ndim == 3 && two_byte && row_block > 512);
-    const bool row_wanted = L->row_even == BOLT_ROW_ON || (L->row_even == BOLT_ROW_AUTO && row_pays);
+    const bool row_wanted = L->row_mode == BOLT_ROW_ON || (L->row_mode == BOLT_ROW_AUTO && row_pays);
     if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
         fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block);
     const long long rows = L->ncell / p.len;
@@ -1101,7 +1101,7 @@ int bolt_lattice_set_row_staging(bolt_lattice* L, int mode) {
     if (!L) return fail(BOLT_EINVAL, "null lattice");
     if (mode != BOLT_ROW_OFF && mode != BOLT_ROW_ON && mode != BOLT_ROW_AUTO)
         return fail(BOLT_EINVAL, "row staging mode must be BOLT_ROW_OFF, _ON or _AUTO");
-    L->row_even = mode;
+    L->row_mode = mode;
     return BOLT_OK;
 }
 
@@ -1809,8 +1809,9 @@ int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
     return BOLT_OK;
 }
 
-// Autotuner: times every (tile width, graph length) candidate over `steps` steps on the
-// batch itself, restores the batch's state afterwards, and keeps the fastest shape.
+// Autotuner: times every (even steps row-staged or direct, tile width, graph length)
+// candidate over `steps` steps on the batch itself, restores the batch's state afterwards,
+// and keeps the fastest.
 int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (steps < 4 || steps % 2) return fail(BOLT_EINVAL, "steps must be even and >= 4");
