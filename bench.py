@@ -43,12 +43,18 @@ COUNTERS = os.path.join("profiles", "hot_path_kernels_hbm_counters.md")
 
 
 def kernel_sources_sha() -> str:
-    """sha256 prefix of the headline kernel's sources (update.hpp then lattice.hpp)."""
+    """sha256 prefix of the headline kernel's code (update.hpp then lattice.hpp) with the
+    comments and whitespace taken out, so that a comment edit does not retire counters
+    taken on the same code."""
     import hashlib
+    import re
     h = hashlib.sha256()
     for name in ("update.hpp", "lattice.hpp"):
-        with open(os.path.join(ROOT, "boltzmann_amd", "csrc", name), "rb") as fh:
-            h.update(fh.read())
+        with open(os.path.join(ROOT, "boltzmann_amd", "csrc", name)) as fh:
+            text = fh.read()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)   # block comments
+        text = re.sub(r"//[^\n]*", " ", text)                 # line comments
+        h.update(" ".join(text.split()).encode())
     return h.hexdigest()[:16]
 
 

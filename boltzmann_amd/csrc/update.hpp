@@ -10,7 +10,9 @@
 //   odd step   load  f_i  = F[x, opp i]            store F[x, i]        = f*_i
 // Every work-item reads and writes the same set of locations and the sets of different
 // cells are disjoint, so one launch updates the array in place with no ordering between
-// work-items.  Wall cells (flag & 1) neither read nor write (full-way bounce-back).
+// work-items.  Wall cells (flag & 1) neither read nor write (full-way bounce-back).  The
+// even step also has a row-staged form (lbm_step_row, below): one work-group per row, every
+// access an aligned row, the shift along the row taken in LDS, with the same bits.
 //
 // Per-cell physics (lib.cl:249-300, 680-717; kernel_gen.py:402-420 for D3Q27-TRT):
 //   rho = sum f, u = sum c f / rho, u += a / omega(+) when an acceleration is active,
