@@ -71,7 +71,7 @@ def ext_path() -> str:
     return os.path.join(HERE, "_boltzmann" + suffix)
 
 
-HIP_UNITS = ("bolt_runtime.hip", "bolt_wide.hip", "bolt_row.hip", "bolt_nn.hip")
+HIP_UNITS = ("bolt_runtime.hip", "bolt_wide.hip", "bolt_wide_keep.hip", "bolt_row.hip", "bolt_nn.hip")
 
 
 def build_hip(extra: list[list[str]] = ()) -> None:

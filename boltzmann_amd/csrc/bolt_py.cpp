@@ -141,6 +141,14 @@ int row_mode(const py::object& mode) {
     return mode.cast<bool>() ? BOLT_ROW_ON : BOLT_ROW_OFF;
 }
 
+// "stream" / "keep" / "auto" -> BOLT_CACHE_STREAM / BOLT_CACHE_KEEP / BOLT_CACHE_AUTO
+int cache_policy(const std::string& policy) {
+    if (policy == "stream") return BOLT_CACHE_STREAM;
+    if (policy == "keep") return BOLT_CACHE_KEEP;
+    if (policy == "auto") return BOLT_CACHE_AUTO;
+    throw BoltError("cache policy: 'stream', 'keep' or 'auto'");
+}
+
 int device_kind(const std::string& dev) {
     std::string u;
     for (char c : dev) u.push_back((char)toupper(c));
@@ -649,6 +657,12 @@ PYBIND11_MODULE(_boltzmann, m) {
             // or "auto" (the default: where it measured faster); results do not change
             check(bolt_lattice_set_row_staging(s.core->lat, row_mode(mode)), "set_row_staging");
         }, py::arg("mode"))
+        .def("set_cache_policy", [](PySimulation& s, const std::string& policy) {
+            // population loads: "stream" (streaming), "keep" (ordinary: the lines stay in the
+            // Infinity Cache) or "auto" (the default: by the step's working set); results do
+            // not change
+            check(bolt_lattice_set_cache_policy(s.core->lat, cache_policy(policy)), "set_cache_policy");
+        }, py::arg("policy"))
         .def_property_readonly("device_bytes", [](PySimulation& s) {
             uint64_t b = 0;
             check(bolt_lattice_device_bytes(s.core->lat, &b), "device_bytes");
@@ -764,12 +778,15 @@ PYBIND11_MODULE(_boltzmann, m) {
         .def("set_row_staging", [](PyBatch& b, py::object mode) {
             check(bolt_batch_set_row_staging(b.core->batch, row_mode(mode)), "set_row_staging");
         }, py::arg("mode"))
+        .def("set_cache_policy", [](PyBatch& b, const std::string& policy) {
+            check(bolt_batch_set_cache_policy(b.core->batch, cache_policy(policy)), "set_cache_policy");
+        }, py::arg("policy"))
         .def("set_active", [](PyBatch& b, py::array_t<bool, py::array::c_style | py::array::forcecast> a) {
             std::vector<uint8_t> v(a.data(), a.data() + a.size());
             check(bolt_batch_set_active(b.core->batch, v.data(), (int)v.size()), "set_active");
         }, py::arg("active"))
         .def("tune", [](PyBatch& b, int steps) {
-            char rep[2048];
+            char rep[8192];  // one line per candidate (36) and the choice
             check(bolt_batch_tune(b.core->batch, steps, rep, sizeof rep), "tune");
             return std::string(rep);
         }, py::arg("steps") = 40)

@@ -81,6 +81,7 @@ struct bolt_lattice {
     int block = 0;
     int vw = 0;                  // cells per lane of the fluid step (0: chosen per lattice)
     int row_mode = BOLT_ROW_AUTO;  // even steps row-staged (lbm_step_row): BOLT_ROW_*
+    int cache_policy = BOLT_CACHE_AUTO;  // population load form: BOLT_CACHE_*
     double* partials = nullptr;  // reduction scratch
     double* result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -102,63 +103,101 @@ static bool is_f64(const bolt_lattice* L) { return L->storage == kF64; }
 static size_t macro_size(const bolt_lattice* L) { return is_f64(L) ? 8 : 4; }
 static int tracer_storage(const bolt_lattice* L) { return is_f64(L) ? kF64 : kF32; }
 
+// The population load form.  MI355X's 256 MB Infinity Cache sits in front of HBM.  A step's
+// working set that fits it is loaded with ordinary loads, so the lines stay for the next step;
+// measured 1.1-1.4x the streaming form on working sets of 150-240 MB (D2Q9 and D3Q27, float
+// and FP16-shifted).  A larger one streams: at 300 MB and more, ordinary loads gained 4 % or
+// lost up to 8 %, and at 1.8 GB they lost 4 %.
+static constexpr uint64_t kCacheFitBytes = 240ull << 20;
+
+static bool loads_stream(int policy, uint64_t working_set) {
+    if (policy == BOLT_CACHE_STREAM) return true;
+    if (policy == BOLT_CACHE_KEEP) return false;
+    return working_set > kCacheFitBytes;
+}
+
+// Bytes one step of the lattice touches: its population planes (fluid and tracers), the
+// flags, every tracer's value and, when a tracer runs as a launch of its own, the density
+// and velocity the fluid step then writes every step.
+static uint64_t step_working_set(const bolt_lattice* L) {
+    const uint64_t nc = (uint64_t)L->ncell;
+    const uint64_t ms = macro_size(L);
+    uint64_t b = nc * ((uint64_t)L->q * store_size(L->storage) + 1);
+    for (const Tracer& t : L->tracers) b += nc * ((uint64_t)t.q * store_size(tracer_storage(L)) + ms);
+    const bool can_fuse = (L->storage == kF32 || L->storage == kF64) && !L->tracers.empty();
+    if (L->tracers.size() > (can_fuse ? 1u : 0u)) b += nc * (uint64_t)(1 + L->ndim) * ms;
+    return b;
+}
+
 // ---------------------------------------------------------------------------------------
 // kernel table
 // ---------------------------------------------------------------------------------------
 using StepFn = void (*)(StepParams);
 
-template <int D, int Q, int C, int S, int TQ>
+// Every kernel comes in two population load forms (update.hpp pop_load): streaming
+// (`stream` true) or ordinary loads.
+template <int D, int Q, int C, int S, int TQ, bool ST>
 static StepFn pick_even(bool even) {
-    return even ? (StepFn)lbm_step<D, Q, C, S, true, TQ> : (StepFn)lbm_step<D, Q, C, S, false, TQ>;
+    return even ? (StepFn)lbm_step<D, Q, C, S, true, TQ, ST> : (StepFn)lbm_step<D, Q, C, S, false, TQ, ST>;
 }
 
-template <int D, int Q, int C, int S>
+template <int D, int Q, int C, int S, bool ST>
 static StepFn step_fn(bool even, int tq) {
-    if (tq == 0) return pick_even<D, Q, C, S, 0>(even);
+    if (tq == 0) return pick_even<D, Q, C, S, 0, ST>(even);
     if constexpr (S == kF32 || S == kF64) {
         constexpr int TQ = D == 2 ? 5 : 7;
-        if (tq == TQ) return pick_even<D, Q, C, S, TQ>(even);
+        if (tq == TQ) return pick_even<D, Q, C, S, TQ, ST>(even);
     }
     return nullptr;
 }
 
-template <int D, int Q, int C>
+template <int D, int Q, int C, bool ST>
 static StepFn step_fn_s(int s, bool even, int tq) {
     switch (s) {
-        case kF32: return step_fn<D, Q, C, kF32>(even, tq);
-        case kF16S: return step_fn<D, Q, C, kF16S>(even, tq);
-        case kBF16S: return step_fn<D, Q, C, kBF16S>(even, tq);
-        case kF64: return step_fn<D, Q, C, kF64>(even, tq);
-        case kF32F: return step_fn<D, Q, C, kF32F>(even, tq);
+        case kF32: return step_fn<D, Q, C, kF32, ST>(even, tq);
+        case kF16S: return step_fn<D, Q, C, kF16S, ST>(even, tq);
+        case kBF16S: return step_fn<D, Q, C, kBF16S, ST>(even, tq);
+        case kF64: return step_fn<D, Q, C, kF64, ST>(even, tq);
+        case kF32F: return step_fn<D, Q, C, kF32F, ST>(even, tq);
     }
     return nullptr;
 }
 
-static StepFn lookup_step(int d, int q, int coll, int s, bool even, int tq) {
-    if (d == 2 && q == 9) return coll == 0 ? step_fn_s<2, 9, 0>(s, even, tq) : step_fn_s<2, 9, 1>(s, even, tq);
+template <bool ST>
+static StepFn step_table(int d, int q, int coll, int s, bool even, int tq) {
+    if (d == 2 && q == 9)
+        return coll == 0 ? step_fn_s<2, 9, 0, ST>(s, even, tq) : step_fn_s<2, 9, 1, ST>(s, even, tq);
     if (d == 3 && q == 27)
-        return coll == 0 ? step_fn_s<3, 27, 0>(s, even, tq) : step_fn_s<3, 27, 1>(s, even, tq);
+        return coll == 0 ? step_fn_s<3, 27, 0, ST>(s, even, tq) : step_fn_s<3, 27, 1, ST>(s, even, tq);
     return nullptr;
 }
 
-// wide fluid steps (update_wide.hpp, instantiated in bolt_wide.hip)
+static StepFn lookup_step(int d, int q, int coll, int s, bool even, int tq, bool stream) {
+    return stream ? step_table<true>(d, q, coll, s, even, tq) : step_table<false>(d, q, coll, s, even, tq);
+}
+
+// wide fluid steps (update_wide.hpp, instantiated in bolt_wide.hip and bolt_wide_keep.hip)
 using WideFn = void (*)(StepParams, WideGeom);
-WideFn bolt_lookup_wide(int d, int q, int coll, int s, int vw, bool even);
+WideFn bolt_lookup_wide(int d, int q, int coll, int s, int vw, bool even, bool stream);
 
 // row-staged even steps (update.hpp lbm_step_row, instantiated in bolt_row.hip)
-StepFn bolt_lookup_row(int d, int q, int coll, int s, int tq, int block);
+StepFn bolt_lookup_row(int d, int q, int coll, int s, int tq, int block, bool stream);
 int bolt_row_slots(int d, int q);
 
-static StepFn lookup_ad(int d, int tq, int s, bool even) {
-    if (d == 2 && tq == 5) {
-        if (s == kF64) return even ? (StepFn)ad_step<2, 5, kF64, true> : (StepFn)ad_step<2, 5, kF64, false>;
-        return even ? (StepFn)ad_step<2, 5, kF32, true> : (StepFn)ad_step<2, 5, kF32, false>;
-    }
-    if (d == 3 && tq == 7) {
-        if (s == kF64) return even ? (StepFn)ad_step<3, 7, kF64, true> : (StepFn)ad_step<3, 7, kF64, false>;
-        return even ? (StepFn)ad_step<3, 7, kF32, true> : (StepFn)ad_step<3, 7, kF32, false>;
-    }
+template <int D, int TQ, int S, bool ST>
+static StepFn ad_fn(bool even) {
+    return even ? (StepFn)ad_step<D, TQ, S, true, ST> : (StepFn)ad_step<D, TQ, S, false, ST>;
+}
+
+template <bool ST>
+static StepFn ad_table(int d, int tq, int s, bool even) {
+    if (d == 2 && tq == 5) return s == kF64 ? ad_fn<2, 5, kF64, ST>(even) : ad_fn<2, 5, kF32, ST>(even);
+    if (d == 3 && tq == 7) return s == kF64 ? ad_fn<3, 7, kF64, ST>(even) : ad_fn<3, 7, kF32, ST>(even);
     return nullptr;
+}
+
+static StepFn lookup_ad(int d, int tq, int s, bool even, bool stream) {
+    return stream ? ad_table<true>(d, tq, s, even) : ad_table<false>(d, tq, s, even);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -915,16 +954,17 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     }
     const size_t first_sep = can_fuse ? 1 : 0;
     const bool sep_tracers = L->tracers.size() > first_sep;
-    StepFn fe = lookup_step(L->ndim, L->q, L->collision, L->storage, true, tq);
-    StepFn fo = lookup_step(L->ndim, L->q, L->collision, L->storage, false, tq);
+    const bool st = loads_stream(L->cache_policy, step_working_set(L));
+    StepFn fe = lookup_step(L->ndim, L->q, L->collision, L->storage, true, tq, st);
+    StepFn fo = lookup_step(L->ndim, L->q, L->collision, L->storage, false, tq, st);
     if (!fe || !fo) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d collision %d storage %d",
                                 L->ndim, L->q, L->collision, L->storage);
     const int vwe = pick_vw(L, can_fuse, true), vwo = pick_vw(L, can_fuse, false);
     const WideGeom wge = make_wide_geom(L->ncell, p.len, vwe), wgo = make_wide_geom(L->ncell, p.len, vwo);
     WideFn we = nullptr, wo = nullptr;
-    if (vwe > 1 && !(we = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true)))
+    if (vwe > 1 && !(we = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true, st)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwe, L->storage);
-    if (vwo > 1 && !(wo = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false)))
+    if (vwo > 1 && !(wo = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwo, false, st)))
         return fail(BOLT_ENOTFOUND, "Kernel not found: %d cells per lane, storage %d", vwo, L->storage);
     // the even step row-staged: one work-group per row of the contiguous axis, which must
     // fit one work-group (1024 cells) and its shifted planes' rows (fluid and fused tracer)
@@ -942,7 +982,7 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
                           !(L->ndim == 3 && two_byte && row_block > 512);
     const bool row_wanted = L->row_mode == BOLT_ROW_ON || (L->row_mode == BOLT_ROW_AUTO && row_pays);
     if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
-        fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block);
+        fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block, st);
     const long long rows = L->ncell / p.len;
     const dim3 grid((unsigned)p.nblocks), blk((unsigned)block);
     const dim3 wgride((unsigned)wge.nblocks), wgrido((unsigned)wgo.nblocks), wblk(256);
@@ -964,7 +1004,7 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
             fill_planes(L, t.g, t.q, tracer_storage(L), pt.gpl);
             pt.tC = t.C;
             pt.t_om = t.omega;
-            StepFn fa = lookup_ad(L->ndim, t.q, tracer_storage(L), even);
+            StepFn fa = lookup_ad(L->ndim, t.q, tracer_storage(L), even, st);
             if (!fa) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d AdvectionDiffusion", L->ndim, t.q);
             hipLaunchKernelGGL(fa, grid, blk, 0, s, pt);
         }
@@ -1102,6 +1142,18 @@ int bolt_lattice_set_row_staging(bolt_lattice* L, int mode) {
     if (mode != BOLT_ROW_OFF && mode != BOLT_ROW_ON && mode != BOLT_ROW_AUTO)
         return fail(BOLT_EINVAL, "row staging mode must be BOLT_ROW_OFF, _ON or _AUTO");
     L->row_mode = mode;
+    return BOLT_OK;
+}
+
+static bool valid_cache_policy(int policy) {
+    return policy == BOLT_CACHE_STREAM || policy == BOLT_CACHE_KEEP || policy == BOLT_CACHE_AUTO;
+}
+
+int bolt_lattice_set_cache_policy(bolt_lattice* L, int policy) {
+    if (!L) return fail(BOLT_EINVAL, "null lattice");
+    if (!valid_cache_policy(policy))
+        return fail(BOLT_EINVAL, "cache policy must be BOLT_CACHE_STREAM, _KEEP or _AUTO");
+    L->cache_policy = policy;
     return BOLT_OK;
 }
 
@@ -1431,7 +1483,7 @@ __device__ inline StepParams batch_step_params(const InstDesc& d, int write_macr
     return p;
 }
 
-template <int COLL, bool EVEN, int TQ>
+template <int COLL, bool EVEN, int TQ, bool STREAM>
 __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
     const Tile t = bp.tiles[b];
@@ -1439,20 +1491,20 @@ __global__ void __launch_bounds__(256) rb_batch_step(const BatchParams bp) {
     const int iy = (int)t.chunk * (int)blockDim.x + (int)threadIdx.x;
     if (iy >= d.ny) return;
     const StepParams p = batch_step_params<TQ>(d, bp.write_macro);
-    update_cell<2, 9, COLL, kF32, EVEN, TQ>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
+    update_cell<2, 9, COLL, kF32, EVEN, TQ, STREAM>(p, t.row, iy, 0, (uint32_t)t.row * (uint32_t)d.ny + iy);
 }
 
 // The even step row-staged (update.hpp row_step): one work-group per (instance, row) tile,
 // launched once per group of tiles whose rows round up to the same multiple of 64 (the
 // group's blockDim.x).  Same per-cell arithmetic as rb_batch_step and the single lattices.
-template <int COLL, int TQ>
+template <int COLL, int TQ, bool STREAM>
 __global__ void __launch_bounds__(1024) rb_batch_row(const BatchParams bp) {
     extern __shared__ __align__(16) unsigned char rb_row_lds_bytes[];
     const int b = xcd_swizzle((int)blockIdx.x, bp.ntiles);
     const Tile t = bp.tiles[b];
     const InstDesc& d = bp.inst[t.inst];
     const StepParams p = batch_step_params<TQ>(d, bp.write_macro);
-    row_step<2, 9, COLL, kF32, TQ>(p, (int)t.row, rb_row_lds_bytes);
+    row_step<2, 9, COLL, kF32, TQ, STREAM>(p, (int)t.row, rb_row_lds_bytes);
 }
 
 struct bolt_batch {
@@ -1469,10 +1521,13 @@ struct bolt_batch {
     std::vector<RowGroup> rgroups;
     int row_mode = BOLT_ROW_AUTO;  // even steps row-staged: BOLT_ROW_* (the autotuner tries both)
     bool rows_whole_lines = false; // every active row fills whole 128-byte lines
+    int cache_policy = BOLT_CACHE_AUTO;  // population load form: BOLT_CACHE_* (tuner: both)
+    uint64_t working_set = 0;      // bytes one step of the active instances touches
     int graph_steps = 100;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0, graph_len = 0;
     bool graph_row = false;
+    bool graph_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t cells = 0;
     bool tracers = true;
@@ -1514,9 +1569,12 @@ static int batch_build_tiles(bolt_batch* B, int block) {
     }
     const size_t nchunk = tiles.size();
     B->rows_whole_lines = true;
-    for (size_t k = 0; k < B->lat.size(); ++k)
-        if ((B->active.empty() || B->active[k]) && (size_t)B->lat[k]->n[1] * sizeof(float) % 128 != 0)
-            B->rows_whole_lines = false;
+    B->working_set = 0;
+    for (size_t k = 0; k < B->lat.size(); ++k) {
+        if (!B->active.empty() && !B->active[k]) continue;
+        if ((size_t)B->lat[k]->n[1] * sizeof(float) % 128 != 0) B->rows_whole_lines = false;
+        B->working_set += step_working_set(B->lat[k]);
+    }
     B->rgroups.clear();
     const bool too_long = !rt[0].empty();
     for (int g = 1; g <= 16 && !too_long; ++g) {
@@ -1554,33 +1612,41 @@ static bool batch_row(const bolt_batch* B) {
     return B->row_mode == BOLT_ROW_ON || (B->row_mode == BOLT_ROW_AUTO && B->rows_whole_lines);
 }
 
-static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
+// Whether the batch's population loads stream (the working set of the active instances
+// against the Infinity Cache, as for single lattices).
+static bool batch_stream(const bolt_batch* B) { return loads_stream(B->cache_policy, B->working_set); }
+
+template <int C, int TQ, bool ST>
+static void batch_launch_t(bolt_batch* B, bool even, int write_macro, hipStream_t s) {
     if (even && batch_row(B)) {
         for (const auto& g : B->rgroups) {
-            BatchParams bp{B->d_inst, B->d_tiles + B->ntiles + g.first, g.count, write_macro ? 1 : 0};
-            const dim3 grid((unsigned)g.count), blk((unsigned)g.block);
-            const size_t lds = batch_row_lds(B, g.block);
-            if (B->tracers) {
-                if (B->collision == BOLT_BGK) hipLaunchKernelGGL((rb_batch_row<0, 5>), grid, blk, lds, s, bp);
-                else hipLaunchKernelGGL((rb_batch_row<1, 5>), grid, blk, lds, s, bp);
-            } else {
-                if (B->collision == BOLT_BGK) hipLaunchKernelGGL((rb_batch_row<0, 0>), grid, blk, lds, s, bp);
-                else hipLaunchKernelGGL((rb_batch_row<1, 0>), grid, blk, lds, s, bp);
-            }
+            BatchParams bp{B->d_inst, B->d_tiles + B->ntiles + g.first, g.count, write_macro};
+            hipLaunchKernelGGL((rb_batch_row<C, TQ, ST>), dim3((unsigned)g.count), dim3((unsigned)g.block),
+                               batch_row_lds(B, g.block), s, bp);
         }
         return;
     }
-    BatchParams bp{B->d_inst, B->d_tiles, B->ntiles, write_macro ? 1 : 0};
+    BatchParams bp{B->d_inst, B->d_tiles, B->ntiles, write_macro};
     const dim3 grid((unsigned)B->ntiles), blk((unsigned)B->block);
-#define BOLT_BATCH_LAUNCH(C, E, TQ) hipLaunchKernelGGL((rb_batch_step<C, E, TQ>), grid, blk, 0, s, bp)
+    if (even) hipLaunchKernelGGL((rb_batch_step<C, true, TQ, ST>), grid, blk, 0, s, bp);
+    else hipLaunchKernelGGL((rb_batch_step<C, false, TQ, ST>), grid, blk, 0, s, bp);
+}
+
+template <int C, int TQ>
+static void batch_launch_c(bolt_batch* B, bool even, int write_macro, hipStream_t s) {
+    if (batch_stream(B)) batch_launch_t<C, TQ, true>(B, even, write_macro, s);
+    else batch_launch_t<C, TQ, false>(B, even, write_macro, s);
+}
+
+static void batch_launch(bolt_batch* B, bool even, bool write_macro, hipStream_t s) {
+    const int wm = write_macro ? 1 : 0;
     if (B->tracers) {
-        if (B->collision == BOLT_BGK) { if (even) BOLT_BATCH_LAUNCH(0, true, 5); else BOLT_BATCH_LAUNCH(0, false, 5); }
-        else { if (even) BOLT_BATCH_LAUNCH(1, true, 5); else BOLT_BATCH_LAUNCH(1, false, 5); }
+        if (B->collision == BOLT_BGK) batch_launch_c<0, 5>(B, even, wm, s);
+        else batch_launch_c<1, 5>(B, even, wm, s);
     } else {
-        if (B->collision == BOLT_BGK) { if (even) BOLT_BATCH_LAUNCH(0, true, 0); else BOLT_BATCH_LAUNCH(0, false, 0); }
-        else { if (even) BOLT_BATCH_LAUNCH(1, true, 0); else BOLT_BATCH_LAUNCH(1, false, 0); }
+        if (B->collision == BOLT_BGK) batch_launch_c<0, 0>(B, even, wm, s);
+        else batch_launch_c<1, 0>(B, even, wm, s);
     }
-#undef BOLT_BATCH_LAUNCH
 }
 
 static int batch_capture(bolt_batch* B, int len) {
@@ -1596,6 +1662,7 @@ static int batch_capture(bolt_batch* B, int len) {
     B->graph_block = B->block;
     B->graph_len = len;
     B->graph_row = batch_row(B);
+    B->graph_stream = batch_stream(B);
     return BOLT_OK;
 }
 
@@ -1607,7 +1674,8 @@ static int batch_enqueue(bolt_batch* B, uint64_t n) {
     const uint64_t len = (uint64_t)B->graph_steps;
     uint64_t done = 0;
     if (len > 0 && n > len + 2) {  // graph_steps 0: every step is a direct launch
-        if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len || B->graph_row != batch_row(B)) {
+        if (!B->graph || B->graph_block != B->block || B->graph_len != (int)len || B->graph_row != batch_row(B) ||
+            B->graph_stream != batch_stream(B)) {
             int rc = batch_capture(B, (int)len);
             if (rc) return rc;
         }
@@ -1798,6 +1866,15 @@ int bolt_batch_set_row_staging(bolt_batch* B, int mode) {
     return BOLT_OK;
 }
 
+int bolt_batch_set_cache_policy(bolt_batch* B, int policy) {
+    if (!B) return fail(BOLT_EINVAL, "null batch");
+    if (!valid_cache_policy(policy))
+        return fail(BOLT_EINVAL, "cache policy must be BOLT_CACHE_STREAM, _KEEP or _AUTO");
+    B->cache_policy = policy;
+    batch_drop_graph(B);
+    return BOLT_OK;
+}
+
 int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (block < 64 || block > 256 || block % 64 != 0)
@@ -1809,9 +1886,9 @@ int bolt_batch_set_shape(bolt_batch* B, int block, int graph_steps) {
     return BOLT_OK;
 }
 
-// Autotuner: times every (even steps row-staged or direct, tile width, graph length)
-// candidate over `steps` steps on the batch itself, restores the batch's state afterwards,
-// and keeps the fastest.
+// Autotuner: times every (population load form, even steps row-staged or direct, tile
+// width, graph length) candidate over `steps` steps on the batch itself, restores the
+// batch's state afterwards, and keeps the fastest.
 int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     if (!B) return fail(BOLT_EINVAL, "null batch");
     if (steps < 4 || steps % 2) return fail(BOLT_EINVAL, "steps must be even and >= 4");
@@ -1837,32 +1914,36 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     }
     std::string rep;
     int best_block = B->block, best_len = B->graph_steps;
-    bool best_row = batch_row(B);
+    bool best_row = batch_row(B), best_stream = batch_stream(B);
     double best = 1e300;
     const int blocks[] = {64, 128, 256};
     const int lens[] = {20, 100, 400};
-    for (int row = 1; row >= 0 && !rc; --row) {
-        rc = bolt_batch_set_row_staging(B, row ? BOLT_ROW_ON : BOLT_ROW_OFF);
-        for (int blk : blocks) {
-            if (rc) break;
-            for (int gl : lens) {
+    for (int stream = 1; stream >= 0 && !rc; --stream) {
+        rc = bolt_batch_set_cache_policy(B, stream ? BOLT_CACHE_STREAM : BOLT_CACHE_KEEP);
+        for (int row = 1; row >= 0 && !rc; --row) {
+            rc = bolt_batch_set_row_staging(B, row ? BOLT_ROW_ON : BOLT_ROW_OFF);
+            for (int blk : blocks) {
                 if (rc) break;
-                rc = bolt_batch_set_shape(B, blk, gl);
-                if (rc) break;
-                double ms = 0;
-                rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);  // warm + capture
-                if (!rc) rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);
-                if (rc) break;
-                const double per = ms / std::max(steps, gl + 4);
-                char line[96];
-                snprintf(line, sizeof line, "block %d graph %d rows %s: %.4f ms/step\n", blk, gl,
-                         row ? "staged" : "direct", per);
-                rep += line;
-                if (per < best) {
-                    best = per;
-                    best_block = blk;
-                    best_len = gl;
-                    best_row = row != 0;
+                for (int gl : lens) {
+                    if (rc) break;
+                    rc = bolt_batch_set_shape(B, blk, gl);
+                    if (rc) break;
+                    double ms = 0;
+                    rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);  // warm + capture
+                    if (!rc) rc = bolt_batch_step_timed(B, (uint64_t)std::max(steps, gl + 4), &ms);
+                    if (rc) break;
+                    const double per = ms / std::max(steps, gl + 4);
+                    char line[112];
+                    snprintf(line, sizeof line, "block %d graph %d rows %s loads %s: %.4f ms/step\n", blk, gl,
+                             row ? "staged" : "direct", stream ? "streaming" : "cached", per);
+                    rep += line;
+                    if (per < best) {
+                        best = per;
+                        best_block = blk;
+                        best_len = gl;
+                        best_row = row != 0;
+                        best_stream = stream != 0;
+                    }
                 }
             }
         }
@@ -1877,10 +1958,11 @@ int bolt_batch_tune(bolt_batch* B, int steps, char* report, size_t len) {
     if (rc) return rc;
     if (e != hipSuccess) return fail(BOLT_EDEVICE, "tune: restoring the batch state failed: %s", hipGetErrorString(e));
     rc = bolt_batch_set_row_staging(B, best_row ? BOLT_ROW_ON : BOLT_ROW_OFF);
+    if (!rc) rc = bolt_batch_set_cache_policy(B, best_stream ? BOLT_CACHE_STREAM : BOLT_CACHE_KEEP);
     if (!rc) rc = bolt_batch_set_shape(B, best_block, best_len);
-    char line[96];
-    snprintf(line, sizeof line, "chosen: block %d graph %d rows %s\n", best_block, best_len,
-             best_row ? "staged" : "direct");
+    char line[112];
+    snprintf(line, sizeof line, "chosen: block %d graph %d rows %s loads %s\n", best_block, best_len,
+             best_row ? "staged" : "direct", best_stream ? "streaming" : "cached");
     rep += line;
     if (report && len) snprintf(report, len, "%s", rep.c_str());
     return rc;

@@ -123,17 +123,18 @@ struct Nbr<3> {
 };
 
 // ---- AA loads and stores ---------------------------------------------------------------
-// Populations are touched once per step, so they are loaded with the streaming
-// (non-temporal) hint: measured on MI355X, D3Q27 TRT 256^3, it lifts FP16-shifted storage
-// from 42.1k to 45.7k MLUPS and float storage by 2 %.  Streaming stores measured slower
-// and are off unless BOLT_NT_STORE is defined; BOLT_NO_NT_LOAD turns the load hint off.
-template <class T>
+// Populations are touched once per step.  STREAM picks the load form, per lattice:
+//  - streaming (non-temporal) loads, for a working set larger than MI355X's 256 MB
+//    Infinity Cache.  Measured on D3Q27 TRT 256^3, they lift FP16-shifted storage from
+//    42.1k to 45.7k MLUPS and float storage by 2-4 %;
+//  - ordinary loads, for a working set that fits that cache.  The lines a step loads and
+//    stores then stay there for the next step: 1.1-1.4x the streaming form on working
+//    sets of 150-240 MB, where the streaming hint keeps them out.
+// Streaming stores measured slower and are off unless BOLT_NT_STORE is defined.
+template <class T, bool STREAM>
 __device__ inline T pop_load(const void* plane, uint32_t cell) {
-#ifndef BOLT_NO_NT_LOAD
-    return __builtin_nontemporal_load(&elem<T>(plane, cell));
-#else
-    return elem<T>(plane, cell);
-#endif
+    if constexpr (STREAM) return __builtin_nontemporal_load(&elem<T>(plane, cell));
+    else return elem<T>(plane, cell);
 }
 
 template <class T>
@@ -150,7 +151,7 @@ __device__ inline void pop_store(void* plane, uint32_t cell, T v) {
 // st<V, T>(k, v) stores it.  CellAA addresses every position in memory (one cell per
 // work-item); the row-staged even step (RowAA, lbm_step_row) keeps the positions shifted
 // along the row in LDS.
-template <int D, bool EVEN>
+template <int D, bool EVEN, bool STREAM>
 struct CellAA {
     void* const* planes;
     const Nbr<D>& nb;
@@ -161,7 +162,7 @@ struct CellAA {
     }
     template <class V, class T>
     __device__ T ld(int k) const {
-        return pop_load<T>(planes[k], at<V>(k));
+        return pop_load<T, STREAM>(planes[k], at<V>(k));
     }
     template <class V, class T>
     __device__ void st(int k, T v) const {
@@ -741,23 +742,23 @@ __device__ inline void update_cell_with(const StepParams& p, const A& a, const G
 }
 
 // One cell of one time step, every AA location addressed in memory (one cell per work-item).
-template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ, bool STREAM>
 __device__ inline void update_cell(const StepParams& p, int ix, int iy, int iz, uint32_t cell) {
     const int flag = elem<uint8_t>((const void*)p.flags, cell);
     if (flag & 1) return;
     const Nbr<D> nb(ix, iy, iz, p);
-    update_cell_with<D, Q, COLL, S, EVEN, TQ>(p, CellAA<D, EVEN>{p.fpl, nb, cell},
-                                              CellAA<D, EVEN>{p.gpl, nb, cell}, flag, cell);
+    update_cell_with<D, Q, COLL, S, EVEN, TQ>(p, CellAA<D, EVEN, STREAM>{p.fpl, nb, cell},
+                                              CellAA<D, EVEN, STREAM>{p.gpl, nb, cell}, flag, cell);
 }
 
-template <int D, int Q, int COLL, int S, bool EVEN, int TQ>
+template <int D, int Q, int COLL, int S, bool EVEN, int TQ, bool STREAM>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(waves_target<Q, S, TQ>())))
     lbm_step(const StepParams p) {
     int ix, iy, iz;
     uint32_t cell;
     if (!locate<D>(p, ix, iy, iz, cell)) return;
-    update_cell<D, Q, COLL, S, EVEN, TQ>(p, ix, iy, iz, cell);
+    update_cell<D, Q, COLL, S, EVEN, TQ, STREAM>(p, ix, iy, iz, cell);
 }
 
 // ---- row-staged even step ----------------------------------------------------------------
@@ -819,7 +820,7 @@ struct RowAA {
 // work-group owns row `row`, work-item z = threadIdx.x its cell z (z < p.len), and `lds`
 // holds (row_slots<fluid> + row_slots<tracer>) * blockDim.x elements of the storage type.
 // Every work-item of the work-group must call it (it holds two barriers).
-template <int D, int Q, int COLL, int S, int TQ>
+template <int D, int Q, int COLL, int S, int TQ, bool STREAM>
 __device__ inline void row_step(const StepParams& p, int row, unsigned char* lds_bytes) {
     using V = VSet<D, Q>;
     using W = VSet<D, TQ != 0 ? TQ : (D == 2 ? 5 : 7)>;  // the tracer's set (unused if TQ == 0)
@@ -847,14 +848,14 @@ __device__ inline void row_step(const StepParams& p, int row, unsigned char* lds
         flag = elem<uint8_t>((const void*)p.flags, cell);
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-            const T v = pop_load<T>(p.fpl[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z);
+            const T v = pop_load<T, STREAM>(p.fpl[k], nb.row_at(-V::c(k, 0), D == 3 ? -V::c(k, 1) : 0) + (uint32_t)z);
             if (V::c(k, D - 1) == 0) own[k] = v;
             else lds[row_slot<V>(k) * stride + z] = v;
         }
         if constexpr (TQ != 0) {
 #pragma unroll
             for (int k = 0; k < TQ; ++k) {
-                const T v = pop_load<T>(p.gpl[k], nb.row_at(-W::c(k, 0), D == 3 ? -W::c(k, 1) : 0) + (uint32_t)z);
+                const T v = pop_load<T, STREAM>(p.gpl[k], nb.row_at(-W::c(k, 0), D == 3 ? -W::c(k, 1) : 0) + (uint32_t)z);
                 if (W::c(k, D - 1) == 0) gown[k] = v;
                 else glds[row_slot<W>(k) * stride + z] = v;
             }
@@ -885,16 +886,16 @@ __device__ inline void row_step(const StepParams& p, int row, unsigned char* lds
 // BMAX: the largest work-group the instantiation serves (256 or 1024).  Launched with
 // p.nblocks = rows, blockDim.x = len rounded up to 64 and (row_slots of the fluid + row_slots
 // of the fused tracer) * blockDim.x elements of dynamic LDS.
-template <int D, int Q, int COLL, int S, int BMAX, int TQ>
+template <int D, int Q, int COLL, int S, int BMAX, int TQ, bool STREAM>
 __global__ void __launch_bounds__(BMAX)
     __attribute__((amdgpu_waves_per_eu(BMAX > 256 ? BMAX / 256 : waves_target<Q, S, TQ>())))
     lbm_step_row(const StepParams p) {
     extern __shared__ __align__(16) unsigned char row_lds_bytes[];
-    row_step<D, Q, COLL, S, TQ>(p, xcd_swizzle((int)blockIdx.x, p.nblocks), row_lds_bytes);
+    row_step<D, Q, COLL, S, TQ, STREAM>(p, xcd_swizzle((int)blockIdx.x, p.nblocks), row_lds_bytes);
 }
 
 // Advection-diffusion tracer on its own, reading the fluid velocity written this step.
-template <int D, int TQ, int S, bool EVEN>
+template <int D, int TQ, int S, bool EVEN, bool STREAM>
 __global__ void __launch_bounds__(256) ad_step(const StepParams p) {
     using W = VSet<D, TQ>;
     int ix, iy, iz;
@@ -906,25 +907,25 @@ __global__ void __launch_bounds__(256) ad_step(const StepParams p) {
     const Nbr<D> nb(ix, iy, iz, p);
     if constexpr (S == kF64) {
         double g[TQ];
-        aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+        aa_load_raw<W, EVEN>(CellAA<D, EVEN, STREAM>{p.gpl, nb, cell}, g);
         double u[3] = {0, 0, 0};
 #pragma unroll
         for (int a = 0; a < D; ++a) u[a] = elem<double>((const void*)p.vel[a], cell);
         double rt;
         f64_ad<W>(g, rt, u, tfixed, (double)p.t_om,
                   tfixed ? elem<double>((const void*)p.tC, cell) : 0.0);
-        aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+        aa_store_raw<W, EVEN>(CellAA<D, EVEN, STREAM>{p.gpl, nb, cell}, g);
         elem<double>(p.tC, cell) = rt;
     } else {
         float g[TQ];
-        aa_load_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+        aa_load_raw<W, EVEN>(CellAA<D, EVEN, STREAM>{p.gpl, nb, cell}, g);
         float u[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int a = 0; a < D; ++a) u[a] = elem<float>((const void*)p.vel[a], cell);
         float rt;
         ref_ad<W>(g, rt, u, tfixed, p.t_om,
                   tfixed ? elem<float>((const void*)p.tC, cell) : 0.0f);
-        aa_store_raw<W, EVEN>(CellAA<D, EVEN>{p.gpl, nb, cell}, g);
+        aa_store_raw<W, EVEN>(CellAA<D, EVEN, STREAM>{p.gpl, nb, cell}, g);
         elem<float>(p.tC, cell) = rt;
     }
 }

@@ -115,10 +115,14 @@ struct Words {
     uint32_t w[W];
 };
 
-template <int W>
+// STREAM: streaming or ordinary load, as update.hpp's pop_load
+template <int W, bool STREAM>
 __device__ inline Words<W> vload(const void* plane, uint32_t elem_byte) {
     using R = typename RawV<W>::T;
-    const R r = __builtin_nontemporal_load((const R*)((const char*)plane + elem_byte));
+    const R* const a = (const R*)((const char*)plane + elem_byte);
+    R r;
+    if constexpr (STREAM) r = __builtin_nontemporal_load(a);
+    else r = *a;
     Words<W> o;
     if constexpr (W == 1) o.w[0] = r;
     else
@@ -265,14 +269,14 @@ struct WideLane {
 // Loads plane `pl` for this step as the lane's packed vector: element k is the value cell
 // z0 + k reads -- even step, the plane's slot at x - c (row shift rdx, rdy and shift
 // dz = -c along the row); odd step, dz = 0 on the own row.
-template <int ES, int VW>
+template <int ES, int VW, bool STREAM>
 __device__ inline Words<VW * ES / 4> wide_load(const void* pl, const WideGeom& g, const WaveTask& t,
                                                const WideLane& l, int rdx, int rdy, int dz) {
     constexpr int W = VW * ES / 4;
     const uint32_t rb = l.rb[rdx + 1][rdy + 1];
     // a lane past the end of the row loads the wavefront's first vector instead (its values
     // are never used), so the loads need no branch
-    Words<W> v = vload<W>(pl, (rb + (uint32_t)(l.active ? l.z0 : t.zb)) * (uint32_t)ES);
+    Words<W> v = vload<W, STREAM>(pl, (rb + (uint32_t)(l.active ? l.z0 : t.zb)) * (uint32_t)ES);
     if (dz < 0) {
         shift_prev<ES, W>(v);
         if (!g.fullrow && l.lane == 0) set_el<ES, W>(v, 0, load_el<ES>(pl, rb + (uint32_t)halo_prev(g, t)));
@@ -423,8 +427,8 @@ constexpr int wide_waves_target() {
 #endif
 }
 
-// The fluid step with VW cells per lane (no fused tracer).
-template <int D, int Q, int COLL, int S, bool EVEN, int VW>
+// The fluid step with VW cells per lane (no fused tracer); STREAM: the population load form.
+template <int D, int Q, int COLL, int S, bool EVEN, int VW, bool STREAM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wide_waves_target<Q, S, VW>())))
     lbm_step_wide(const StepParams p, const WideGeom g) {
     static_assert(VW == 2 || VW == 4, "2 or 4 cells per lane");
@@ -474,7 +478,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wide_w
         // even: f_i = F[x - c_i, i]; odd: f_i = F[x, opp i]
         int rdx, rdy, dz;
         plane_shift<D, V, EVEN>(i, rdx, rdy, dz);
-        fr[i] = wide_load<ES, VW>(p.fpl[EVEN ? i : V::opp(i)], g, t, l, rdx, rdy, dz);
+        fr[i] = wide_load<ES, VW, STREAM>(p.fpl[EVEN ? i : V::opp(i)], g, t, l, rdx, rdy, dz);
     }
 
     A r[VW], u[VW][3], vv[VW], omp[VW], omn[VW];

@@ -215,6 +215,20 @@ int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
 #define BOLT_ROW_ON 1
 #define BOLT_ROW_AUTO 2
 int bolt_lattice_set_row_staging(bolt_lattice* lat, int mode);
+/* Launch tuning (no counterpart in the reference): how the steps load the populations.
+ *   BOLT_CACHE_STREAM  streaming (non-temporal) loads, which leave the lines out of the GPU's
+ *                      caches: the better form when a step's working set exceeds the 256 MB
+ *                      Infinity Cache of MI355X
+ *   BOLT_CACHE_KEEP    ordinary loads: the lines stay in the Infinity Cache for the next step,
+ *                      1.1-1.4x faster where the working set fits it
+ *   BOLT_CACHE_AUTO    (the default) KEEP when a step's working set is at most 240 MiB
+ *                      (populations, flags, tracer values and any per-step density and
+ *                      velocity), STREAM otherwise
+ * Results do not depend on the choice (bit-identical). */
+#define BOLT_CACHE_STREAM 0
+#define BOLT_CACHE_KEEP 1
+#define BOLT_CACHE_AUTO 2
+int bolt_lattice_set_cache_policy(bolt_lattice* lat, int policy);
 
 /* ---- batches: many small 2D Rayleigh-Benard-type lattices in one launch per step -----
  * No counterpart in the reference (it runs one Simulation per process).  Instance k is a
@@ -253,14 +267,18 @@ int bolt_batch_set_shape(bolt_batch* b, int block, int graph_steps);
  * never (every step through the tiles of set_shape).  bolt_batch_tune tries on and off.
  * Results do not depend on the choice. */
 int bolt_batch_set_row_staging(bolt_batch* b, int mode);
+/* Population loads as bolt_lattice_set_cache_policy, for the working set of all active
+ * instances together; bolt_batch_tune tries both forms. */
+int bolt_batch_set_cache_policy(bolt_batch* b, int policy);
 /* instances that later steps advance (active[n], non-zero: advance); an inactive instance
  * keeps its state and can still be read, reduced or checked for its residual.  Change it
  * only between step calls (each of an even count, so every instance stays in its natural
  * population layout). */
 int bolt_batch_set_active(bolt_batch* b, const uint8_t* active, int n);
 /* autotuner: times the candidate shapes on this batch (state restored afterwards) -- work-
- * group width 64/128/256, 20/100/400 steps per graph, even steps row-staged or not --, keeps
- * the fastest and writes a per-candidate report into report[len]. */
+ * group width 64/128/256, 20/100/400 steps per graph, even steps row-staged or not,
+ * streaming or ordinary population loads --, keeps the fastest and writes a per-candidate
+ * report into report[len]. */
 int bolt_batch_tune(bolt_batch* b, int steps, char* report, size_t len);
 
 /* ---- warm-start network (no counterpart in the reference) ----------------------------
