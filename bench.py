@@ -434,8 +434,9 @@ def nn_throughput(nx: int = 512, ny: int = 256, batch: int = 8, width: int = 16,
 
 def profile_kernels(steps: int = 20) -> dict:
     """A short run of every hot-path kernel for a profiler: the 256^3 D3Q27 TRT lattice in
-    every storage format, the 4096 x 1024 D2Q9 TRT cylinder channel in f32 and f16s, a
-    1024 x 1024 D2Q9 lattice with a fused D2Q5 tracer and Boussinesq coupling, and the
+    every storage format, the 4096 x 1024 D2Q9 TRT cylinder channel in f32 and f16s (with the
+    ordinary loads its working set gets by default, and f32 once more with streaming loads),
+    a 1024 x 1024 D2Q9 lattice with a fused D2Q5 tracer and Boussinesq coupling, and the
     64-cell Rayleigh-Benard batch with direct launches (no graph replay), its even steps
     row-staged and then direct (the autotuner's two choices).  Returns the launches and the
     algorithmic bytes per launch of each workload."""
@@ -450,15 +451,16 @@ def profile_kernels(steps: int = 20) -> dict:
     del rho, vel
     flags, vel0 = cylinder_flow()
     nu = 0.05 * 128 / 1000.0
-    for st, b in (("f32", 73), ("f16s", 37)):
+    for st, b, policy in (("f32", 73, "auto"), ("f16s", 37, "auto"), ("f32", 73, "stream")):
         sim = boltzmann_amd.Simulation("gpu", [4096, 1024], Omega.TRT(1 / (3 * nu + 0.5), 1 / 0.52),
                                        storage=st)
+        sim.set_cache_policy(policy)
         sim.cells.flags[...] = flags
         sim.fluid.vel[...] = vel0
         sim.step(2)
         sim.step_timed(10 * steps)
-        out[f"cylinder_4096x1024_d2q9_trt_{st}"] = {"launches": 10 * steps + 2,
-                                                    "bytes_per_launch": 4096 * 1024 * b}
+        key = f"cylinder_4096x1024_d2q9_trt_{st}" + ("_streaming" if policy == "stream" else "")
+        out[key] = {"launches": 10 * steps + 2, "bytes_per_launch": 4096 * 1024 * b}
         del sim
     m = 1024
     sim = boltzmann_amd.Simulation("gpu", [m, m], Omega.TRT(1 / 0.6, 1 / 0.9))

@@ -33,6 +33,10 @@
 // shift27 ceilings: planes 2^26 bytes apart (the unpadded 256^3 lattice) against planes
 // padded by 256 B to 1 MiB, and the 27 planes interleaved in blocks of 64-1024 cells.  Every index is
 // checked against the element count it addresses.
+//
+// `bw_probe REPS cache` times the inplace27 mix on lattices of 2^17 to 2^24 cells (14 MB to
+// 1.8 GB of planes) with ordinary and with streaming loads: where the planes fit MI355X's
+// 256 MB Infinity Cache, ordinary loads leave the lines there for the next pass.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -361,6 +365,29 @@ int main(int argc, char** argv) {
     const double lat = 2.0 * 4.0 * (double)total;
     const double half = 2.0 * 2.0 * (double)total;
     auto best = [](double a, double c) { return a > c ? a : c; };
+
+    if (argc > 2 && !strcmp(argv[2], "cache")) {
+        // planes n apart, 27 * n <= total floats for every n below; more passes on the small
+        // lattices, so that each figure times at least ~2^29 cells
+        printf("{\"plane_mb\": [");
+        for (int k = 17; k <= 24; ++k) printf("%s%.1f", k > 17 ? ", " : "", 27.0 * 4 * (1ll << k) / 1e6);
+        for (int nt = 0; nt < 2; ++nt) {
+            printf("], \"%s\": [", nt ? "inplace27_streaming" : "inplace27_ordinary");
+            for (int k = 17; k <= 24; ++k) {
+                const long long m = 1ll << k;
+                const int r = (int)(reps * std::max(1ll, (1ll << 24) / m));
+                const double by = 2.0 * 4.0 * 27 * (double)m;
+                const double v = nt ? gbs([&] { inplace_planes<27, float, true><<<grid(m), b>>>(f, m, m); }, by, r)
+                                    : gbs([&] { inplace_planes<27, float, false><<<grid(m), b>>>(f, m, m); }, by, r);
+                printf("%s%.1f", k > 17 ? ", " : "", v);
+                fflush(stdout);
+            }
+        }
+        printf("]}\n");
+        CK(hipFree(f));
+        CK(hipFree(sink));
+        return 0;
+    }
 
     if (argc > 2 && !strcmp(argv[2], "layout")) {
         // inplace27 and shift27 with the planes st = n + pad floats apart (the unpadded
