@@ -991,6 +991,7 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     for (uint64_t it = 0; it < n; ++it) {
         const bool even = (it % 2) == 0;
         p.write_macro = (it + 1 == n) || sep_tracers;
+        p.reverse = even ? 0 : 1;  // the odd steps walk the tiles last to first
         if (even && fr) {
             StepParams pr = p;
             pr.nblocks = (int)rows;
@@ -1480,6 +1481,7 @@ __device__ inline StepParams batch_step_params(const InstDesc& d, int write_macr
     p.write_macro = write_macro;
     p.tC = global_ptr(d.C);
     p.t_om = d.om_g;
+    p.reverse = 0;  // the batch kernels map work-groups to tiles themselves
     return p;
 }
 

@@ -56,6 +56,10 @@ struct StepParams {
     void* gpl[7];
     void* tC;
     float t_om;
+    // Visit the launch's tiles last to first.  The odd steps do, so that each step starts on
+    // the lines the previous one touched last, which the Infinity Cache still holds (the AA
+    // steps are independent of the order in which cells are visited).
+    int reverse;
 };
 
 template <class T>
@@ -578,7 +582,8 @@ __device__ inline void ref_ad(float (&g)[V::q], float& r, const float (&u)[3], b
 // Maps the work-item to its cell.  Returns false for work-items past the end of a row.
 template <int D>
 __device__ inline bool locate(const StepParams& p, int& ix, int& iy, int& iz, uint32_t& cell) {
-    const int b = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    int b = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    if (p.reverse) b = p.nblocks - 1 - b;
     const int row = b / p.cpr;
     const int chunk = b - row * p.cpr;
     const int zc = chunk * (int)blockDim.x + (int)threadIdx.x;
@@ -891,7 +896,9 @@ __global__ void __launch_bounds__(BMAX)
     __attribute__((amdgpu_waves_per_eu(BMAX > 256 ? BMAX / 256 : waves_target<Q, S, TQ>())))
     lbm_step_row(const StepParams p) {
     extern __shared__ __align__(16) unsigned char row_lds_bytes[];
-    row_step<D, Q, COLL, S, TQ, STREAM>(p, xcd_swizzle((int)blockIdx.x, p.nblocks), row_lds_bytes);
+    int row = xcd_swizzle((int)blockIdx.x, p.nblocks);
+    if (p.reverse) row = p.nblocks - 1 - row;
+    row_step<D, Q, COLL, S, TQ, STREAM>(p, row, row_lds_bytes);
 }
 
 // Advection-diffusion tracer on its own, reading the fluid velocity written this step.

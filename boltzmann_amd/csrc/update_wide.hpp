@@ -436,9 +436,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wide_w
     using A = typename WideConv<S>::A;
     // the wavefront's task, row and neighbour rows are the same on every lane: computed
     // once from a wavefront-uniform index (scalar registers)
-    const int task = __builtin_amdgcn_readfirstlane(
+    int task = __builtin_amdgcn_readfirstlane(
         xcd_swizzle((int)blockIdx.x, g.nblocks) * 4 + (int)(threadIdx.x >> 6));
     if (task >= g.tasks) return;  // a whole wavefront
+    if (p.reverse) task = g.tasks - 1 - task;
     const WaveTask t = wave_task(g, task);
     WideLane l;
     l.lane = (int)(threadIdx.x & 63);
