@@ -991,7 +991,11 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     for (uint64_t it = 0; it < n; ++it) {
         const bool even = (it % 2) == 0;
         p.write_macro = (it + 1 == n) || sep_tracers;
-        p.reverse = even ? 0 : 1;  // the odd steps walk the tiles last to first
+        // a lattice larger than the Infinity Cache walks its odd steps' tiles last to first,
+        // so that each step starts on the lines the previous one left in that cache
+        // (measured: +0.6 to +1.9 % f32 and +3.8 to +6 % f16s on D3Q27 256^3); a lattice that
+        // fits it keeps one order (reversing cost the cache-resident D2Q9 lattices 1-4 %)
+        p.reverse = (st && !even) ? 1 : 0;
         if (even && fr) {
             StepParams pr = p;
             pr.nblocks = (int)rows;

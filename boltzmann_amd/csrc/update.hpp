@@ -56,9 +56,10 @@ struct StepParams {
     void* gpl[7];
     void* tC;
     float t_om;
-    // Visit the launch's tiles last to first.  The odd steps do, so that each step starts on
-    // the lines the previous one touched last, which the Infinity Cache still holds (the AA
-    // steps are independent of the order in which cells are visited).
+    // Visit the launch's tiles last to first.  The odd steps of a lattice larger than the
+    // Infinity Cache do, so that each step starts on the lines the previous one touched last,
+    // which that cache still holds (an AA step's result does not depend on the order in
+    // which cells are visited).
     int reverse;
 };
 
