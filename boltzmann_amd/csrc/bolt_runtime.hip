@@ -106,7 +106,7 @@ static int tracer_storage(const bolt_lattice* L) { return is_f64(L) ? kF64 : kF3
 // The population load form.  MI355X's 256 MB Infinity Cache sits in front of HBM.  A step's
 // working set that fits it is loaded with ordinary loads, so the lines stay for the next step;
 // measured 1.1-1.4x the streaming form on working sets of 150-240 MB (D2Q9 and D3Q27, float
-// and FP16-shifted).  A larger one streams: at 300 MB and more, ordinary loads gained 4 % or
+// and FP16-shifted).  A larger one streams: at 268-319 MB ordinary loads gained up to 4 % or
 // lost up to 8 %, and at 1.8 GB they lost 4 %.
 static constexpr uint64_t kCacheFitBytes = 240ull << 20;
 

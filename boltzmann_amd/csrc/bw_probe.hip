@@ -37,6 +37,10 @@
 // `bw_probe REPS cache` times the inplace27 mix on lattices of 2^17 to 2^24 cells (14 MB to
 // 1.8 GB of planes) with ordinary and with streaming loads: where the planes fit MI355X's
 // 256 MB Infinity Cache, ordinary loads leave the lines there for the next pass.
+//
+// `bw_probe REPS march` times a step pair of the inplace27 mix on the 256^3 lattice as two
+// full passes, and as a march over K = 8, 16, 32 slabs of consecutive cells: pass one over
+// slab s, then pass two over slab s - 1, so that pass two finds its slab's lines fresh.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -365,6 +369,35 @@ int main(int argc, char** argv) {
     const double lat = 2.0 * 4.0 * (double)total;
     const double half = 2.0 * 2.0 * (double)total;
     auto best = [](double a, double c) { return a > c ? a : c; };
+
+    if (argc > 2 && !strcmp(argv[2], "march")) {
+        // slab s of every plane: cells [s * m, (s + 1) * m), addressed as a lattice of m cells
+        // whose planes lie n apart (inplace_planes with the base moved to the slab)
+        const double pair = 2.0 * lat;
+        for (int nt = 1; nt >= 0; --nt) {
+            auto pass = [&](long long c0, long long m) {
+                if (nt) inplace_planes<27, float, true><<<grid(m), b>>>(f + c0, m, n);
+                else inplace_planes<27, float, false><<<grid(m), b>>>(f + c0, m, n);
+            };
+            printf("%s\"%s\": {\"full\": %.1f", nt ? "{" : ", ", nt ? "streaming" : "ordinary",
+                   gbs([&] { pass(0, n); pass(0, n); }, pair, reps));
+            for (int K : {8, 16, 32}) {
+                const long long m = n / K;  // n is a multiple of 32
+                printf(", \"march%d\": %.1f", K, gbs([&] {
+                    for (int s = 0; s <= K; ++s) {
+                        if (s < K) pass(s * m, m);
+                        if (s >= 1) pass((s - 1) * m, m);
+                    }
+                }, pair, reps));
+                fflush(stdout);
+            }
+            printf("}");
+        }
+        printf("}\n");
+        CK(hipFree(f));
+        CK(hipFree(sink));
+        return 0;
+    }
 
     if (argc > 2 && !strcmp(argv[2], "cache")) {
         // planes n apart, 27 * n <= total floats for every n below; more passes on the small
