@@ -480,11 +480,17 @@ static int pick_block(const bolt_lattice* L) {
 // against 378.6); everything else runs one cell per lane, whose even (streaming) step and
 // float/double odd step were the fastest (f32 730.1 / 675.6 us against 938.9 / 702.9 at 2
 // cells and 802.8 / 706.8 at 4).  A fused tracer always runs one cell per lane.
-static int pick_vw(const bolt_lattice* L, bool fused_tracer, bool even) {
+// Cells per lane of the fluid step.  The built-in choice (vw 0): 2 on the odd steps of the
+// 2-byte formats, except for D2Q9 lattices whose population loads keep their lines in the
+// Infinity Cache (`stream` false), which run one cell per lane (measured 0.4-8.7 % faster on
+// 1024x1024 to 8192x1536 lattices; the D3Q27 ones gained or lost up to 4 % and keep 2);
+// one cell per lane otherwise.
+static int pick_vw(const bolt_lattice* L, bool fused_tracer, bool even, bool stream) {
     if (fused_tracer) return 1;
     const int len = L->n[L->ndim - 1];
     if (L->vw == 0) {
         const bool half = L->storage == kF16S || L->storage == kBF16S;
+        if (half && L->ndim == 2 && !stream) return 1;
         return (half && !even && len % 2 == 0 && len >= 64) ? 2 : 1;
     }
     if (L->vw <= 1) return 1;
@@ -959,7 +965,7 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     StepFn fo = lookup_step(L->ndim, L->q, L->collision, L->storage, false, tq, st);
     if (!fe || !fo) return fail(BOLT_ENOTFOUND, "Kernel not found: D%dQ%d collision %d storage %d",
                                 L->ndim, L->q, L->collision, L->storage);
-    const int vwe = pick_vw(L, can_fuse, true), vwo = pick_vw(L, can_fuse, false);
+    const int vwe = pick_vw(L, can_fuse, true, st), vwo = pick_vw(L, can_fuse, false, st);
     const WideGeom wge = make_wide_geom(L->ncell, p.len, vwe), wgo = make_wide_geom(L->ncell, p.len, vwo);
     WideFn we = nullptr, wo = nullptr;
     if (vwe > 1 && !(we = bolt_lookup_wide(L->ndim, L->q, L->collision, L->storage, vwe, true, st)))
@@ -978,8 +984,11 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     // (+0.2 to +10.6 %; rows of 1000 float cells lost 2 %), except D3Q27 rows over 512 cells
     // in the 2-byte formats, whose 1024-work-item groups fit one per CU (-11 %)
     const bool two_byte = L->storage == kF16S || L->storage == kBF16S;
+    // nor D2Q9 lattices in the 2-byte formats that fit the Infinity Cache, whose direct even
+    // step measured 0.2-4.2 % faster than the row-staged one
     const bool row_pays = (size_t)p.len * store_size(L->storage) % 128 == 0 &&
-                          !(L->ndim == 3 && two_byte && row_block > 512);
+                          !(L->ndim == 3 && two_byte && row_block > 512) &&
+                          !(L->ndim == 2 && two_byte && !st);
     const bool row_wanted = L->row_mode == BOLT_ROW_ON || (L->row_mode == BOLT_ROW_AUTO && row_pays);
     if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
         fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block, st);

@@ -195,7 +195,8 @@ int bolt_lattice_set_block(bolt_lattice* lat, int block);
 /* Launch tuning: cells of the contiguous axis per lane in the fluid step (1, 2 or 4; 2 at
  * most for double storage; the axis must be a multiple of it), on every step; 0 restores
  * the built-in choice, measured fastest on MI355X: 2 cells per lane on the odd (own-cell)
- * steps of the 2-byte formats, one cell per lane otherwise.  Wider lanes make every
+ * steps of the 2-byte formats, except D2Q9 lattices that fit the Infinity Cache (see
+ * bolt_lattice_set_cache_policy), one cell per lane otherwise.  Wider lanes make every
  * population access an aligned 4-, 8- or 16-byte vector and do the streaming shift along
  * the row with lane rotations.  A tracer fused into the fluid step always runs one cell per
  * lane.  Results do not depend on the choice (bit-identical). */
@@ -209,7 +210,8 @@ int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
  *   BOLT_ROW_ON    wherever it can run
  *   BOLT_ROW_OFF   never
  *   BOLT_ROW_AUTO  (the default) where it measured faster: rows that fill whole 128-byte
- *                  cache lines, except D3Q27 rows over 512 cells in the 2-byte formats
+ *                  cache lines, except D3Q27 rows over 512 cells in the 2-byte formats and
+ *                  D2Q9 lattices in the 2-byte formats that fit the Infinity Cache
  * Results do not depend on the choice (bit-identical). */
 #define BOLT_ROW_OFF 0
 #define BOLT_ROW_ON 1
