@@ -984,11 +984,10 @@ static int enqueue_steps(bolt_lattice* L, uint64_t n) {
     // (+0.2 to +10.6 %; rows of 1000 float cells lost 2 %), except D3Q27 rows over 512 cells
     // in the 2-byte formats, whose 1024-work-item groups fit one per CU (-11 %)
     const bool two_byte = L->storage == kF16S || L->storage == kBF16S;
-    // nor D2Q9 lattices in the 2-byte formats that fit the Infinity Cache, whose direct even
-    // step measured 0.2-4.2 % faster than the row-staged one
+    // nor D2Q9 lattices that fit the Infinity Cache, whose direct even step measured as fast
+    // in float storage and 0.2-6.4 % faster in the 2-byte formats and with a fused tracer
     const bool row_pays = (size_t)p.len * store_size(L->storage) % 128 == 0 &&
-                          !(L->ndim == 3 && two_byte && row_block > 512) &&
-                          !(L->ndim == 2 && two_byte && !st);
+                          !(L->ndim == 3 && two_byte && row_block > 512) && !(L->ndim == 2 && !st);
     const bool row_wanted = L->row_mode == BOLT_ROW_ON || (L->row_mode == BOLT_ROW_AUTO && row_pays);
     if (row_wanted && vwe == 1 && row_block <= 1024 && row_lds <= 65536)
         fr = bolt_lookup_row(L->ndim, L->q, L->collision, L->storage, tq, row_block, st);

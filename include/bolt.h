@@ -211,7 +211,7 @@ int bolt_lattice_set_vector_width(bolt_lattice* lat, int vw);
  *   BOLT_ROW_OFF   never
  *   BOLT_ROW_AUTO  (the default) where it measured faster: rows that fill whole 128-byte
  *                  cache lines, except D3Q27 rows over 512 cells in the 2-byte formats and
- *                  D2Q9 lattices in the 2-byte formats that fit the Infinity Cache
+ *                  D2Q9 lattices that fit the Infinity Cache
  * Results do not depend on the choice (bit-identical). */
 #define BOLT_ROW_OFF 0
 #define BOLT_ROW_ON 1
