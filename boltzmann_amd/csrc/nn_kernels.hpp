@@ -23,12 +23,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16;
 
-__device__ inline u16 bf16_bits(float v) {
-    uint32_t u = __float_as_uint(v);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (u16)(u >> 16);
-}
+// bfloat16 bits of v, rounded to nearest even (NaN kept quiet): gfx950's conversion
+// instruction, the same bits as the integer form u + 0x7fff + ((u >> 16) & 1) >> 16 for every
+// float
+__device__ inline u16 bf16_bits(float v) { return __builtin_bit_cast(u16, (__bf16)v); }
 __device__ inline float bf16_float(u16 b) { return __uint_as_float(((uint32_t)b) << 16); }
 
 enum : int { kMK = 0, kKM = 1, kMKconv = 2 };   // A forms
