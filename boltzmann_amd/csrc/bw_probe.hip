@@ -41,6 +41,12 @@
 // `bw_probe REPS march` times a step pair of the inplace27 mix on the 256^3 lattice as two
 // full passes, and as a march over K = 8, 16, 32 slabs of consecutive cells: pass one over
 // slab s, then pass two over slab s - 1, so that pass two finds its slab's lines fresh.
+//
+// `bw_probe REPS sync` prices the step boundary of a persistent kernel against launches: the
+// time of one grid-wide barrier (cooperative launch, every work-group resident, cooperative
+// groups' grid sync) against one launch of an empty kernel replayed from a HIP graph and
+// one issued directly on a stream.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -333,6 +339,17 @@ __global__ void __launch_bounds__(256) inplace_phased(T* __restrict__ f, long lo
     for (int q = 0; q < 27; ++q) f[q * n + i] = v[(q + 1) % 27];
 }
 
+// iters grid-wide barriers (launched cooperatively: every work-group is resident)
+__global__ void grid_barriers(int iters, int* sink) {
+    cooperative_groups::grid_group g = cooperative_groups::this_grid();
+    for (int i = 0; i < iters; ++i) g.sync();
+    if (iters < 0) sink[0] = 1;  // never taken: keeps the argument used
+}
+
+__global__ void empty_kernel(int* sink) {
+    if (sink == nullptr) return;
+}
+
 template <typename F>
 static double gbs(F launch, double bytes, int reps) {
     hipEvent_t a, b;
@@ -369,6 +386,69 @@ int main(int argc, char** argv) {
     const double lat = 2.0 * 4.0 * (double)total;
     const double half = 2.0 * 2.0 * (double)total;
     auto best = [](double a, double c) { return a > c ? a : c; };
+
+    if (argc > 2 && !strcmp(argv[2], "sync")) {
+        int dev = 0, cus = 0, per = 0;
+        CK(hipGetDevice(&dev));
+        CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, grid_barriers, 256, 0));
+        const int blocks = cus * std::min(per, 8);
+        const int iters = 200 * reps;
+        int* isink = (int*)sink;
+        hipEvent_t e0, e1;
+        CK(hipEventCreate(&e0));
+        CK(hipEventCreate(&e1));
+        auto coop = [&](int nit) {
+            void* args[] = {(void*)&nit, (void*)&isink};
+            CK(hipLaunchCooperativeKernel((const void*)grid_barriers, dim3(blocks), dim3(256), args, 0, 0));
+        };
+        coop(10);
+        CK(hipDeviceSynchronize());
+        float ms1 = 0.f, ms0 = 0.f;
+        CK(hipEventRecord(e0, 0));
+        coop(iters);
+        CK(hipEventRecord(e1, 0));
+        CK(hipEventSynchronize(e1));
+        CK(hipEventElapsedTime(&ms1, e0, e1));
+        CK(hipEventRecord(e0, 0));
+        coop(0);
+        CK(hipEventRecord(e1, 0));
+        CK(hipEventSynchronize(e1));
+        CK(hipEventElapsedTime(&ms0, e0, e1));
+        // the same number of empty launches, from a captured graph and directly on a stream
+        hipStream_t st;
+        CK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        const int K = 400;
+        hipGraph_t gr;
+        hipGraphExec_t ge;
+        CK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < K; ++k) empty_kernel<<<blocks, 256, 0, st>>>(isink);
+        CK(hipStreamEndCapture(st, &gr));
+        CK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+        CK(hipGraphLaunch(ge, st));
+        CK(hipStreamSynchronize(st));
+        float msg = 0.f, mss = 0.f;
+        CK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) CK(hipGraphLaunch(ge, st));
+        CK(hipEventRecord(e1, st));
+        CK(hipEventSynchronize(e1));
+        CK(hipEventElapsedTime(&msg, e0, e1));
+        CK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps * K; ++r) empty_kernel<<<blocks, 256, 0, st>>>(isink);
+        CK(hipEventRecord(e1, st));
+        CK(hipEventSynchronize(e1));
+        CK(hipEventElapsedTime(&mss, e0, e1));
+        CK(hipGetLastError());
+        printf("{\"work_groups\": %d, \"grid_barrier_us\": %.2f, \"graph_launch_us\": %.2f, "
+               "\"stream_launch_us\": %.2f}\n",
+               blocks, (ms1 - ms0) * 1e3 / iters, msg * 1e3 / (reps * K), mss * 1e3 / (reps * K));
+        CK(hipGraphExecDestroy(ge));
+        CK(hipGraphDestroy(gr));
+        CK(hipStreamDestroy(st));
+        CK(hipFree(f));
+        CK(hipFree(sink));
+        return 0;
+    }
 
     if (argc > 2 && !strcmp(argv[2], "march")) {
         // slab s of every plane: cells [s * m, (s + 1) * m), addressed as a lattice of m cells
